@@ -1,0 +1,211 @@
+/*
+ * ibp.h -- C ABI of libibp, the MI355X-native Invariant Bit Packing library.
+ *
+ * Plain pointers and sizes only (no torch types).  Every entry point returns an
+ * ibp_status; on failure ibp_last_error() (thread-local) holds the message.
+ * Work is enqueued on the caller's HIP stream (`stream` is a hipStream_t; NULL
+ * is the null stream).  Unless stated otherwise a call does not synchronise.
+ *
+ * Element width `elem_bytes` is 1, 2, 4 or 8 (the slot codec takes 4 or 8).
+ * Index arrays hold `index_bytes` = 4 (int32) or 8 (int64) signed entries.
+ * Data pointers may be device memory or pinned (hipHostMalloc) host memory.
+ *
+ * Each declaration names the reference interface it replaces
+ * (AKKamath/InvariantBitPacking, file:line).
+ */
+#ifndef IBP_H
+#define IBP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum ibp_status {
+    IBP_OK = 0,
+    IBP_ERR_INVALID = 1,     /* bad argument (the reference's TORCH_CHECK failures) */
+    IBP_ERR_HIP = 2,         /* HIP runtime error (the reference exits the process) */
+    IBP_ERR_NOMEM = 3,       /* allocation failed / admission impossible */
+    IBP_ERR_UNSUPPORTED = 4, /* valid request outside what this build implements */
+    IBP_ERR_BUSY = 5         /* serving: request not finished yet */
+} ibp_status;
+
+/* Where a data pointer lives; IBP_LOC_AUTO asks the HIP runtime. */
+typedef enum ibp_loc { IBP_LOC_AUTO = 0, IBP_LOC_DEVICE = 1, IBP_LOC_HOST = 2 } ibp_loc;
+
+/* ---- library state --------------------------------------------------------- */
+
+/* Thread-local description of the last failure (empty string if none). */
+const char *ibp_last_error(void);
+
+/* Library version string. */
+const char *ibp_version(void);
+
+/* Replaces print_debug_msg (src/ibp_api.cu:9-12; ibp/__init__.py:8-9). */
+void ibp_set_debug(int flag);
+
+/* Number of rows that device kernels skipped because their index was out of
+ * range since the last reset (the reference reads out of bounds instead).  The
+ * counter lives in device memory; reading it synchronises the device. */
+int ibp_index_errors(uint64_t *count, int reset);
+
+/* ---- preprocess -------------------------------------------------------------- */
+
+/* Replaces preprocess (src/preprocess.cu:17-76) -> ibp::preproc_data
+ * (include/preproc/ibp_preproc_host.cuh:24-115).  data [num_vecs x vec_size];
+ * threshold < 0 runs the 0.7..1.0 sweep.  mask/bitval [vec_size] (device) are
+ * written only when some threshold saves bits: the caller zero-initialises them
+ * (the reference returns torch::zeros in that case).  chosen_step (device
+ * int32, optional) receives the sweep index picked or -1.  workspace: device
+ * memory of ibp_preprocess_workspace_size() bytes, or NULL to let the library
+ * allocate (stream-ordered).  Asynchronous. */
+size_t ibp_preprocess_workspace_size(int elem_bytes, int64_t vec_size, float threshold);
+int ibp_preprocess(int elem_bytes, const void *data, int64_t num_vecs, int64_t vec_size,
+                   float threshold, void *mask, void *bitval, int32_t *chosen_step,
+                   void *workspace, size_t workspace_bytes, void *stream);
+
+/* Replaces preprocess_kmeans (src/preprocess.cu:88-139) -> ibp::preproc_kmeans
+ * (include/preproc/ibp_preproc_host.cuh:119-284).  masks/bitvals [K x V]
+ * (device), clusters [num_vecs] int32 (device or pinned host).  seed >= 0
+ * re-seeds the libc generator with srand(seed) before the random_shuffle
+ * initialisation; seed < 0 continues the process-wide rand() state like the
+ * reference.  threshold is accepted and ignored, as in the reference.
+ * Synchronises the stream before returning. */
+int ibp_preprocess_kmeans(int elem_bytes, const void *data, int64_t num_vecs, int64_t vec_size,
+                          int num_clusters, float threshold, int64_t seed, void *masks,
+                          void *bitvals, int32_t *clusters, void *stream);
+
+/* ---- compressed size and pack ----------------------------------------------- */
+
+/* Replaces get_compress_size (src/compress.cu:25-174) ->
+ * check_compress_size_kernel (include/misc/ibp_misc_kernels.cuh:9-32).
+ * sizes [n] int64 (device or pinned host).  index NULL means rows 0..n-1.
+ * total (device uint64, optional) is ADDED to, never reset.  Asynchronous. */
+int ibp_get_compress_size(int elem_bytes, const void *data, int64_t num_vecs, int64_t vec_size,
+                          const void *mask, const void *bitval, const void *index,
+                          int index_bytes, int64_t n, int64_t *sizes, uint64_t *total,
+                          void *stream);
+
+/* Replaces compress_inplace (src/compress.cu:176-236) -> ibp::compress_inplace
+ * (include/compress/ibp_compress_host.cuh:8-27).  Slot i of `output` receives
+ * row index[i] (or row i) of `input`, packed when that makes it smaller.
+ * output == input gives the in-place form; with an index the two must not
+ * alias (the reference races there).  bitmask [ceil(n/32)] int32 (device) is
+ * fully written, no pre-zeroing needed.  Asynchronous. */
+int ibp_compress(int elem_bytes, const void *input, void *output, int64_t num_vecs,
+                 int64_t vec_size, const void *mask, const void *bitval, const void *index,
+                 int index_bytes, int64_t n, int32_t *bitmask, void *stream);
+
+/* ---- fetch (decompress) ------------------------------------------------------- */
+
+/* Replaces decompress_fetch (src/decompress.cu:10-156) -> ibp::decompress_fetch
+ * (include/decompress/ibp_decompress_host.cuh:9-168).
+ * out[k] = unpack(comp[index[k]]) for k < n (index NULL: n rows in order).
+ * comp [num_vecs x vec_size] may be pinned host memory (read over PCIe inside
+ * the kernel) or device memory.  comp_len: expected compressed row length in
+ * elements (<= 0 means vec_size), a performance hint only.  impl: -1 auto,
+ * 0/1/5 unpack (identical output), 2 raw gather without unpack, 3/4 the
+ * reference's transfer-only baselines (wrong output by design).  nblks/nthreads
+ * <= 0 pick defaults.  Out-of-range indices produce zero rows and are counted
+ * (ibp_index_errors).  No allocation, no host synchronisation: safe under
+ * stream capture. */
+int ibp_decompress_fetch(int elem_bytes, void *out, const void *comp, int64_t num_vecs,
+                         int64_t vec_size, const void *mask, const void *bitval,
+                         const int32_t *bitmask, const void *index, int index_bytes, int64_t n,
+                         int comp_len, int nblks, int nthreads, int impl, int comp_loc,
+                         void *stream);
+
+/* ---- batched fetch (no reference counterpart: BASELINE's batched form) ------ */
+
+/* One table fetch of a batch: fields as in ibp_decompress_fetch. */
+typedef struct ibp_fetch_task {
+    const void *comp;
+    const void *mask;
+    const void *bitval;
+    const int32_t *bitmask;
+    const void *index; /* NULL: rows 0..n-1 */
+    void *out;
+    int64_t num_vecs;
+    int64_t vec_size;
+    int64_t n;
+    int32_t elem_bytes;
+    int32_t index_bytes;
+    int32_t comp_len; /* hint, <= 0 means vec_size */
+    int32_t comp_loc; /* ibp_loc */
+} ibp_fetch_task;
+
+typedef struct ibp_plan ibp_plan;
+
+/* Builds a plan for a fixed set of tasks (pointers are captured: refill the
+ * index/output buffers in place between launches).  The plan owns a device work
+ * queue; all tasks run in ONE persistent kernel launch. */
+int ibp_plan_create(const ibp_fetch_task *tasks, int num_tasks, ibp_plan **plan);
+int ibp_plan_destroy(ibp_plan *plan);
+/* Launch on a stream (graph-capturable: no allocation, no memset node). */
+int ibp_plan_launch(ibp_plan *plan, void *stream);
+/* Times candidate launch configurations on `stream` (runs the plan several
+ * times) and keeps the fastest; returns its mean time in *best_ms. */
+int ibp_plan_autotune(ibp_plan *plan, void *stream, float *best_ms);
+/* Captures the launch into a HIP graph owned by the plan; replays with
+ * ibp_plan_graph_launch. */
+int ibp_plan_capture_graph(ibp_plan *plan, void *stream);
+int ibp_plan_graph_launch(ibp_plan *plan, void *stream);
+/* Describes the chosen configuration (rows per work item, grid, ...). */
+int ibp_plan_describe(ibp_plan *plan, char *buf, size_t len);
+
+/* ---- serving (no reference counterpart: BASELINE's served form) -------------- */
+
+typedef struct ibp_server ibp_server;
+
+typedef struct ibp_server_config {
+    size_t hbm_budget_bytes; /* output arena size (admission by free HBM) */
+    int num_streams;         /* concurrent batches on the device */
+    int max_batch_requests;  /* requests merged into one launch */
+    int64_t max_batch_rows;  /* rows merged into one launch */
+} ibp_server_config;
+
+int ibp_server_create(const ibp_server_config *cfg, ibp_server **srv);
+int ibp_server_destroy(ibp_server *srv);
+/* Registers a compressed tensor set; returns its id in *set_id. */
+int ibp_server_register(ibp_server *srv, int elem_bytes, const void *comp, int64_t num_vecs,
+                        int64_t vec_size, const void *mask, const void *bitval,
+                        const int32_t *bitmask, int comp_len, int comp_loc, int *set_id);
+/* Submits a request: fetch rows `index` (int64, host memory, copied at submit)
+ * of set `set_id`.  Returns immediately with *req_id; the request is admitted
+ * when its output fits the arena. */
+int ibp_server_submit(ibp_server *srv, int set_id, const int64_t *index, int64_t n,
+                      int64_t *req_id);
+/* Waits for a request (timeout_ms < 0: forever).  On success *out points to
+ * the [n x vec_size] result in the device arena, valid until
+ * ibp_server_release(req_id); *latency_us is arrival -> completion. */
+int ibp_server_wait(ibp_server *srv, int64_t req_id, int timeout_ms, void **out,
+                    double *latency_us);
+int ibp_server_release(ibp_server *srv, int64_t req_id);
+/* Statistics since creation: completed requests, launches, rows, p50/p99
+ * latency (us), peak arena use (bytes), admissions that had to wait. */
+typedef struct ibp_server_stats {
+    int64_t completed;
+    int64_t launches;
+    int64_t rows;
+    double p50_us;
+    double p99_us;
+    double mean_batch_requests;
+    size_t peak_arena_bytes;
+    int64_t waited_admissions;
+} ibp_server_stats;
+int ibp_server_stats_get(ibp_server *srv, ibp_server_stats *st);
+/* Test hook: pretend the arena has only `bytes` free (0 disables). */
+int ibp_server_set_budget_override(ibp_server *srv, size_t bytes);
+
+/* ---- self-test of the element-level bit arithmetic on the host ------------ */
+/* Runs the codec's element functions (compiled for host) against a brute-force
+ * bit-by-bit model on `trials` random cases; returns IBP_OK or IBP_ERR_INVALID
+ * with the first mismatch in ibp_last_error(). */
+int ibp_selftest_bitops(int trials, uint64_t seed);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* IBP_H */

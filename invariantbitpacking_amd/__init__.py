@@ -1,0 +1,273 @@
+"""Invariant Bit Packing on MI355X: the `ibp` Python API.
+
+Drop-in for the reference module `ibp` (ibp/__init__.py:1-41) and its
+extension `ibp_cuda` (src/ibp_api.cu:124-135): same function names, argument
+order, defaults, return values and error type (RuntimeError).  Every call
+goes through the C ABI of libibp.so (include/ibp.h); torch supplies device
+memory, pinned host memory and the current HIP stream only.
+
+    import invariantbitpacking_amd as ibp
+    mask, bitval = ibp.preprocess(data, 0.9)
+    bitmask = ibp.compress_inplace(pinned, mask, bitval)
+    rows = ibp.decompress_fetch(pinned, mask, bitval, bitmask, torch.device("cuda"), index_arr=idx)
+"""
+import ctypes
+
+import torch
+
+from . import _capi
+from ._capi import IBPError  # noqa: F401  (re-exported)
+
+__all__ = ["print_debug", "preprocess", "preprocess_kmeans", "get_compress_size", "get_single_compress_len",
+           "compress_inplace", "compress", "decompress_fetch", "IBPError", "version", "index_errors"]
+
+_SIGNED = {1: torch.int8, 2: torch.int16, 4: torch.int32, 8: torch.int64}
+
+
+def _ptr(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _accessible(t):
+    """On a HIP device or in pinned host memory (the reference's TORCH_CHECK)."""
+    return t.is_cuda or (t.device.type == "cpu" and t.is_pinned())
+
+
+def _check(cond, msg):
+    if not cond:
+        raise RuntimeError(msg)
+
+
+def _loc(t):
+    return 1 if t.is_cuda else 2
+
+
+def _index_arg(index_arr, require_cuda):
+    """Returns (tensor kept alive, element bytes) for an index array."""
+    if index_arr is None:
+        return None, 0
+    if require_cuda:
+        _check(index_arr.is_cuda, "Index array must accessible by CUDA device (GPU or pinned memory)")
+    else:
+        _check(_accessible(index_arr), "Index array must accessible by CUDA device (GPU or pinned memory)")
+    _check(index_arr.dim() == 1, "Index array should be 1D")
+    idx = index_arr
+    if idx.dtype not in (torch.int32, torch.int64):
+        idx = idx.to(torch.int64)
+    idx = idx.contiguous()
+    return idx, idx.element_size()
+
+
+def version():
+    return _capi.lib().ibp_version().decode()
+
+
+def print_debug(flag):
+    """Turn the library's debug messages on or off (ibp/__init__.py:8-9)."""
+    _capi.lib().ibp_set_debug(1 if flag else 0)
+
+
+def index_errors(reset=False):
+    """Rows skipped by kernels because their index was out of range (synchronises)."""
+    v = ctypes.c_uint64(0)
+    _capi.call("ibp_index_errors", ctypes.byref(v), 1 if reset else 0)
+    return v.value
+
+
+def preprocess(dataset, threshold=None):
+    """mask, bitval [V] (signed int of the element width, on the GPU).
+
+    src/preprocess.cu:17-76.  threshold None sweeps 0.7..1.0 and keeps the
+    threshold that saves the most bits; both outputs stay zero if none saves any.
+    Asynchronous, on the current stream.
+    """
+    _check(_accessible(dataset), "Input tensor must accessible by CUDA device (GPU or pinned memory)")
+    _check(dataset.dim() == 2, "Input tensor must be 2D [num_vecs x vec_size]")
+    w = dataset.element_size()
+    _check(w in (1, 2, 4, 8), "Input tensor must be 1, 2, 4, or 8-byte datatype")
+    _check(dataset.is_contiguous(), "Input tensor must be contiguous")
+    t = -1.0 if threshold is None else float(threshold)
+    n, v = dataset.shape
+    mask = torch.zeros(v, dtype=_SIGNED[w], device="cuda")
+    bitval = torch.zeros(v, dtype=_SIGNED[w], device="cuda")
+    ws_bytes = _capi.lib().ibp_preprocess_workspace_size(w, v, t)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device="cuda")
+    _capi.call("ibp_preprocess", w, _ptr(dataset), n, v, t, _ptr(mask), _ptr(bitval), None, _ptr(ws), ws_bytes,
+               _stream())
+    return mask, bitval
+
+
+def preprocess_kmeans(dataset, centroids, threshold=None, *, seed=None):
+    """masks [K,V], bitvals [K,V] (GPU) and clusters [N] int32 (pinned host).
+
+    src/preprocess.cu:88-139.  `threshold` is ignored, as in the reference.
+    `seed` (extension): re-seed libc rand() before the random initialisation;
+    None continues the process-wide rand() state like the reference.
+    """
+    _check(_accessible(dataset), "Input tensor must accessible by CUDA device (GPU or pinned memory)")
+    _check(dataset.dim() == 2, "Input tensor must be 2D [num_vecs x vec_size]")
+    w = dataset.element_size()
+    _check(w in (4, 8), "Input tensor must be 4-byte or 8-byte datatype")
+    _check(dataset.is_contiguous(), "Input tensor must be contiguous")
+    k = int(centroids)
+    n, v = dataset.shape
+    masks = torch.zeros((k, v), dtype=_SIGNED[w], device="cuda")
+    bitvals = torch.zeros((k, v), dtype=_SIGNED[w], device="cuda")
+    clusters = torch.zeros(n, dtype=torch.int32).pin_memory()
+    t = -1.0 if threshold is None else float(threshold)
+    _capi.call("ibp_preprocess_kmeans", w, _ptr(dataset), n, v, k, t, -1 if seed is None else int(seed),
+               _ptr(masks), _ptr(bitvals), _ptr(clusters), _stream())
+    return masks, bitvals, clusters
+
+
+def _mask_check(dataset, mask, bitval):
+    _check(_accessible(mask), "Mask tensor must accessible by CUDA device (GPU or pinned memory)")
+    _check(_accessible(bitval), "Bitval tensor must accessible by CUDA device (GPU or pinned memory)")
+    w = dataset.element_size()
+    _check(mask.element_size() == w and bitval.element_size() == w,
+           "Mask and bitval tensors must have the same datatype size (4-byte or 8-byte) as input dataset")
+    _check(mask.is_contiguous() and bitval.is_contiguous(), "Mask and bitval tensors must be contiguous")
+    v = dataset.size(0) if dataset.dim() == 1 else dataset.size(1)
+    _check(mask.numel() >= v and bitval.numel() >= v, "Mask and bitval must hold at least vec_size elements")
+
+
+def _shape(dataset):
+    return (1, dataset.size(0)) if dataset.dim() == 1 else (dataset.size(0), dataset.size(1))
+
+
+def get_compress_size(dataset, mask, bitval, index_arr=None, compress_total=None):
+    """Compressed bytes of each row (int64 [n]; src/compress.cu:25-174).
+
+    compress_total (1-element int64 tensor) is incremented by the total and
+    never reset.  Asynchronous unless compress_total lives in host memory.
+    """
+    _check(_accessible(dataset), "Input tensor must accessible by CUDA device (GPU or pinned memory)")
+    _check(dataset.dim() in (1, 2), "Input tensor must be 1D [vec_size] or 2D [num_vecs x vec_size]")
+    _check(dataset.is_contiguous(), "Input tensor must be contiguous")
+    w = dataset.element_size()
+    _check(w in (1, 2, 4, 8), "Input tensor must be 1, 2, 4, or 8-byte datatype")
+    _mask_check(dataset, mask, bitval)
+    num_vecs, v = _shape(dataset)
+    idx, ib = _index_arg(index_arr, require_cuda=False)
+    n = num_vecs if idx is None else idx.size(0)
+    total_dev = None
+    if compress_total is not None:
+        _check(_accessible(compress_total), "Compress ctr must accessible by CUDA device (GPU or pinned memory)")
+        _check(compress_total.dim() == 1 and compress_total.size(0) == 1, "Compress ctr should be a single element")
+        if compress_total.is_cuda and compress_total.dtype == torch.int64:
+            total_dev = compress_total
+        else:
+            total_dev = torch.zeros(1, dtype=torch.int64, device="cuda")
+    if dataset.is_cuda:
+        sizes = torch.empty(n, dtype=torch.int64, device=dataset.device)
+    else:
+        sizes = torch.empty(n, dtype=torch.int64, pin_memory=True)
+    _capi.call("ibp_get_compress_size", w, _ptr(dataset), num_vecs, v, _ptr(mask), _ptr(bitval), _ptr(idx), ib, n,
+               _ptr(sizes), _ptr(total_dev), _stream())
+    if compress_total is not None and total_dev is not compress_total:
+        compress_total += total_dev.to(compress_total.device).to(compress_total.dtype)
+    return sizes
+
+
+def get_single_compress_len(dataset, mask, bitval, index_arr=None):
+    """Average compressed row length in elements (ibp/__init__.py:22-29).
+    index_arr is accepted and ignored, as in the reference."""
+    comp_sizes = get_compress_size(dataset, mask, bitval)
+    torch.cuda.synchronize()
+    og_size = dataset.numel() * dataset.element_size()
+    comp_size = int(torch.sum(comp_sizes))
+    return int(comp_size / og_size * dataset.size(1))
+
+
+def compress_inplace(dataset, mask, bitval, index_arr=None):
+    """Packs rows in place; returns the int32 bitmask [ceil(n/32)] of packed rows
+    (src/compress.cu:176-236).  With index_arr, slot i receives row index_arr[i]
+    (rows are gathered first, so the result does not depend on execution order).
+    """
+    _check(_accessible(dataset), "Input tensor must accessible by CUDA device (GPU or pinned memory)")
+    _check(dataset.dim() in (1, 2), "Input tensor must be 1D [vec_size] or 2D [num_vecs x vec_size]")
+    _check(dataset.is_contiguous(), "Input tensor must be contiguous")
+    w = dataset.element_size()
+    _check(w in (4, 8), "Input tensor must be 4-byte or 8-byte datatype")
+    _mask_check(dataset, mask, bitval)
+    num_vecs, v = _shape(dataset)
+    idx, _ = _index_arg(index_arr, require_cuda=False)
+    n = num_vecs if idx is None else idx.size(0)
+    bitmask = torch.empty((n + 31) // 32, dtype=torch.int32, device="cuda")
+    src = dataset
+    if idx is not None:
+        _check(n <= num_vecs, "index_arr longer than the dataset")
+        rows = dataset.view(num_vecs, v)
+        src = rows.index_select(0, idx.to(rows.device).long()).contiguous()
+        if not src.is_cuda:
+            src = src.pin_memory()
+    _capi.call("ibp_compress", w, _ptr(src), _ptr(dataset), n, v, _ptr(mask), _ptr(bitval), None, 0, n,
+               _ptr(bitmask), _stream())
+    if src is not dataset:
+        torch.cuda.current_stream().synchronize()  # keep the gathered rows alive until packed
+    return bitmask
+
+
+def compress(dataset, mask, bitval, index_arr=None):
+    """The reference's `ibp.compress` calls a binding that does not exist
+    (ibp/__init__.py:35-36 vs src/ibp_api.cu:131)."""
+    raise NotImplementedError("ibp.compress (condensed stream) is not part of this build")
+
+
+def decompress_fetch(comp_dataset, mask, bitval, bitmask, device, comp_len=None, index_arr=None, nblks=None,
+                     nthds=None, imp=None, output_tensor=None):
+    """Rows of the compressed dataset, unpacked (src/decompress.cu:10-156).
+
+    out[k] = row index_arr[k] (all rows in order without an index).  The
+    compressed data may be pinned host memory (read over PCIe inside the
+    kernel) or GPU memory.  comp_len is a hint (expected compressed row length
+    in elements); imp selects the reference's variants (-1/0/1/5 unpack, 2 raw
+    gather, 3/4 transfer-only baselines).  Asynchronous on the current stream.
+    """
+    _check(_accessible(comp_dataset), "Input tensor must accessible by CUDA device (GPU or pinned memory)")
+    _check(comp_dataset.dim() in (1, 2),
+           "Input tensor must be 1D [vec_size] or 2D [num_vecs x vec_size] got %d" % comp_dataset.dim())
+    _check(comp_dataset.is_contiguous(), "Input tensor must be contiguous")
+    w = comp_dataset.element_size()
+    _check(w in (4, 8), "Input tensor must be 4-byte or 8-byte datatype")
+    _mask_check(comp_dataset, mask, bitval)
+    _check(bitmask.is_cuda or _accessible(bitmask), "Bitmask tensor must accessible by CUDA device")
+    _check(bitmask.element_size() == 4 and bitmask.is_contiguous(), "Bitmask must be a contiguous int32 tensor")
+    num_vecs, v = _shape(comp_dataset)
+    _check(bitmask.numel() >= (num_vecs + 31) // 32, "Bitmask too short for the dataset")
+    idx, ib = _index_arg(index_arr, require_cuda=True)
+    n = num_vecs if idx is None else idx.size(0)
+    device = torch.device(device)
+    cl = 0
+    if comp_len is not None:
+        cl = int(comp_len)
+        cl = cl if cl > 0 else 0
+    if output_tensor is not None:
+        _check(output_tensor.device == device or (device.type == "cuda" and output_tensor.is_cuda and
+                                                  device.index is None),
+               "Output tensor device does not match specified output device")
+        _check(output_tensor.dtype == comp_dataset.dtype, "Output tensor dtype does not match input dataset dtype")
+        _check(output_tensor.dim() == 2 and output_tensor.size(0) == n and output_tensor.size(1) == v,
+               "Output tensor shape does not match expected decompressed shape")
+        _check(output_tensor.is_contiguous(), "Output tensor must be contiguous")
+        out = output_tensor
+        if not out.is_cuda:
+            _check(out.is_pinned(), "Output tensor on the host must be pinned")
+    elif device.type == "cuda":
+        out = torch.empty((n, v), dtype=comp_dataset.dtype, device=device)
+    else:
+        out = torch.empty((n, v), dtype=comp_dataset.dtype, pin_memory=True)
+    _capi.call("ibp_decompress_fetch", w, _ptr(out), _ptr(comp_dataset), num_vecs, v, _ptr(mask), _ptr(bitval),
+               _ptr(bitmask), _ptr(idx), ib, n, cl, 0 if nblks is None else int(nblks),
+               0 if nthds is None else int(nthds), -1 if imp is None else int(imp), _loc(comp_dataset), _stream())
+    return out
+
+
+from .batched import BatchedFetch  # noqa: E402
+from .serve import Server  # noqa: E402
+
+__all__ += ["BatchedFetch", "Server"]

@@ -1,0 +1,78 @@
+"""Batched fetch: many tables fetched and unpacked by ONE persistent kernel.
+
+No reference counterpart (the reference fetches one table per call); each
+table's result is exactly what decompress_fetch returns for it.  The plan
+captures pointers, so index and output buffers are refilled in place between
+launches; `capture_graph()` records the launch into a HIP graph.
+"""
+import ctypes
+
+import torch
+
+from . import _capi
+
+
+class BatchedFetch:
+    """tasks: list of dicts with keys comp, mask, bitval, bitmask, out and
+    optionally index (int32/int64 GPU tensor), comp_len (int)."""
+
+    def __init__(self, tasks):
+        self._keep = []
+        arr = (_capi.FetchTask * len(tasks))()
+        for i, t in enumerate(tasks):
+            comp, out = t["comp"], t["out"]
+            index = t.get("index")
+            num_vecs, v = (1, comp.size(0)) if comp.dim() == 1 else tuple(comp.shape)
+            n = num_vecs if index is None else index.numel()
+            if out.shape != (n, v) or out.dtype != comp.dtype or not out.is_contiguous():
+                raise RuntimeError("task %d: output must be a contiguous [%d, %d] %s tensor" % (i, n, v, comp.dtype))
+            if index is not None:
+                if not index.is_cuda or index.dim() != 1:
+                    raise RuntimeError("task %d: index must be a 1-D GPU tensor" % i)
+                if index.dtype not in (torch.int32, torch.int64):
+                    index = index.to(torch.int64)
+                index = index.contiguous()
+            self._keep += [comp, t["mask"], t["bitval"], t["bitmask"], out, index]
+            arr[i] = _capi.FetchTask(
+                comp.data_ptr(), t["mask"].data_ptr(), t["bitval"].data_ptr(), t["bitmask"].data_ptr(),
+                None if index is None else index.data_ptr(), out.data_ptr(), num_vecs, v, n,
+                comp.element_size(), 0 if index is None else index.element_size(), int(t.get("comp_len") or 0),
+                1 if comp.is_cuda else 2)
+        self._tasks = arr
+        self._plan = ctypes.c_void_p()
+        _capi.call("ibp_plan_create", arr, len(tasks), ctypes.byref(self._plan))
+        self.tasks = tasks
+
+    @staticmethod
+    def _stream():
+        return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def launch(self):
+        _capi.call("ibp_plan_launch", self._plan, self._stream())
+
+    def autotune(self):
+        ms = ctypes.c_float(0)
+        _capi.call("ibp_plan_autotune", self._plan, self._stream(), ctypes.byref(ms))
+        return ms.value
+
+    def capture_graph(self):
+        _capi.call("ibp_plan_capture_graph", self._plan, self._stream())
+
+    def graph_launch(self):
+        _capi.call("ibp_plan_graph_launch", self._plan, self._stream())
+
+    def describe(self):
+        buf = ctypes.create_string_buffer(512)
+        _capi.call("ibp_plan_describe", self._plan, buf, 512)
+        return buf.value.decode()
+
+    def close(self):
+        if self._plan:
+            _capi.lib().ibp_plan_destroy(self._plan)
+            self._plan = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

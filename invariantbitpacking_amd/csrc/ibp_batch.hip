@@ -1,0 +1,307 @@
+// ibp_batch.hip -- batched fetch: many tables of uneven row counts and widths
+// fetched and unpacked by ONE persistent kernel from a device-side work queue.
+//
+// No reference counterpart (the reference issues one decompress_fetch per
+// table, e.g. tests/dlrm_comp.py:79-81); values are those of
+// ibp_decompress_fetch on each table.
+//
+// Work item = (task, range of rows).  Waves claim items with one atomic on the
+// queue head; the head is reset by a small kernel launched before the
+// persistent kernel on the same stream, so a launch is two kernel nodes and can
+// be captured into a HIP graph (no memset node).
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "ibp_batch.h"
+#include "ibp_internal.h"
+
+namespace ibp {
+
+__global__ void batch_reset_kernel(unsigned long long *head)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *head = 0;
+}
+
+template <int ROWS>
+__device__ __forceinline__ void run_item(const TaskDev &t, int64_t b, int64_t e)
+{
+    const FetchParams &p = t.p;
+    const bool i32 = p.index != nullptr && t.index_bytes == 4;
+    if (t.small) {
+        if (t.elem_bytes == 8) {
+            const SmallCols<uint64_t> col = load_small_cols<uint64_t>(p.mask, p.bitval, (int)p.vec_size);
+            for (int64_t base = b; base < e; base += ROWS) {
+                FetchParams q = p;
+                q.n = e; // rows beyond the item end are not this wave's
+                if (i32) fetch_small_group<uint64_t, int32_t, ROWS>(q, col, base);
+                else fetch_small_group<uint64_t, int64_t, ROWS>(q, col, base);
+            }
+        } else {
+            const SmallCols<uint32_t> col = load_small_cols<uint32_t>(p.mask, p.bitval, (int)p.vec_size);
+            for (int64_t base = b; base < e; base += ROWS) {
+                FetchParams q = p;
+                q.n = e;
+                if (i32) fetch_small_group<uint32_t, int32_t, ROWS>(q, col, base);
+                else fetch_small_group<uint32_t, int64_t, ROWS>(q, col, base);
+            }
+        }
+    } else {
+        for (int64_t i = b; i < e; ++i) {
+            if (t.elem_bytes == 8) {
+                if (i32) fetch_large_row<uint64_t, int32_t>(p, i);
+                else fetch_large_row<uint64_t, int64_t>(p, i);
+            } else {
+                if (i32) fetch_large_row<uint32_t, int32_t>(p, i);
+                else fetch_large_row<uint32_t, int64_t>(p, i);
+            }
+        }
+    }
+}
+
+template <int ROWS>
+__global__ __launch_bounds__(256) void batch_kernel(const TaskDev *tasks, int ntasks, int64_t total_items,
+                                                    unsigned long long *head)
+{
+    const int lane = lane_id();
+    for (;;) {
+        unsigned long long claimed = 0;
+        if (lane == 0) claimed = atomicAdd(head, 1ull);
+        const int64_t item = readfirstlane((int64_t)claimed);
+        if (item >= total_items) break; // every wave reaches this exit
+        // Task of the item: last task whose item_begin <= item.
+        int lo = 0, hi = ntasks - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) / 2;
+            if (tasks[mid].item_begin <= item) lo = mid;
+            else hi = mid - 1;
+        }
+        const TaskDev t = tasks[lo];
+        const int64_t b = (item - t.item_begin) * t.rows_per_item;
+        int64_t e = b + t.rows_per_item;
+        if (e > t.p.n) e = t.p.n;
+        run_item<ROWS>(t, b, e);
+    }
+}
+
+int64_t fill_tasks(const ibp_fetch_task *tasks, int n, const BatchConfig &cfg, TaskDev *out)
+{
+    int64_t items = 0;
+    for (int i = 0; i < n; ++i) {
+        const ibp_fetch_task &t = tasks[i];
+        TaskDev d;
+        memset(&d, 0, sizeof(d));
+        d.p.out = t.out;
+        d.p.comp = t.comp;
+        d.p.mask = t.mask;
+        d.p.bitval = t.bitval;
+        d.p.bitmask = (const uint32_t *)t.bitmask;
+        d.p.index = t.index;
+        d.p.num_vecs = t.num_vecs;
+        d.p.vec_size = t.vec_size;
+        d.p.n = t.n;
+        d.p.first_words = first_words_for(t.vec_size, t.elem_bytes, t.comp_len);
+        d.p.err = t.index ? index_error_counter() : nullptr;
+        d.elem_bytes = t.elem_bytes;
+        d.index_bytes = t.index_bytes;
+        d.small = t.vec_size <= kWave;
+        d.rows_per_item = d.small ? cfg.rows_per_item_small : cfg.rows_per_item_large;
+        d.item_begin = items;
+        items += (t.n + d.rows_per_item - 1) / d.rows_per_item;
+        out[i] = d;
+    }
+    return items;
+}
+
+int launch_batch(const TaskDev *dev_tasks, int ntasks, int64_t total_items, unsigned long long *head,
+                 const BatchConfig &cfg, hipStream_t s)
+{
+    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, s, head);
+    const dim3 grid((unsigned)cfg.blocks), block(256);
+    switch (cfg.rows_in_flight) {
+    case 2: hipLaunchKernelGGL((batch_kernel<2>), grid, block, 0, s, dev_tasks, ntasks, total_items, head); break;
+    case 8: hipLaunchKernelGGL((batch_kernel<8>), grid, block, 0, s, dev_tasks, ntasks, total_items, head); break;
+    default: hipLaunchKernelGGL((batch_kernel<4>), grid, block, 0, s, dev_tasks, ntasks, total_items, head); break;
+    }
+    IBP_LAUNCH_CHECK();
+    return IBP_OK;
+}
+
+} // namespace ibp
+
+using namespace ibp;
+
+struct ibp_plan {
+    std::vector<ibp_fetch_task> tasks;
+    std::vector<TaskDev> host_tasks;
+    TaskDev *dev_tasks = nullptr;
+    unsigned long long *head = nullptr;
+    int64_t total_items = 0;
+    BatchConfig cfg{32, 2, 4, 0};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    hipStream_t capture_stream = nullptr;
+    int device = -1;
+};
+
+namespace {
+
+int rebuild(ibp_plan *pl)
+{
+    pl->host_tasks.resize(pl->tasks.size());
+    pl->total_items = fill_tasks(pl->tasks.data(), (int)pl->tasks.size(), pl->cfg, pl->host_tasks.data());
+    IBP_HIP(hipMemcpy(pl->dev_tasks, pl->host_tasks.data(), sizeof(TaskDev) * pl->host_tasks.size(),
+                      hipMemcpyHostToDevice));
+    if (pl->graph_exec) {
+        (void)hipGraphExecDestroy(pl->graph_exec);
+        pl->graph_exec = nullptr;
+    }
+    if (pl->graph) {
+        (void)hipGraphDestroy(pl->graph);
+        pl->graph = nullptr;
+    }
+    return IBP_OK;
+}
+
+int launch(ibp_plan *pl, hipStream_t s)
+{
+    return launch_batch(pl->dev_tasks, (int)pl->host_tasks.size(), pl->total_items, pl->head, pl->cfg, s);
+}
+
+} // namespace
+
+extern "C" int ibp_plan_create(const ibp_fetch_task *tasks, int num_tasks, ibp_plan **plan)
+{
+    IBP_CHECK(plan != nullptr, "null plan pointer");
+    IBP_CHECK(num_tasks >= 1 && tasks != nullptr, "a plan needs at least one task");
+    for (int i = 0; i < num_tasks; ++i) {
+        const ibp_fetch_task &t = tasks[i];
+        IBP_CHECK(t.elem_bytes == 4 || t.elem_bytes == 8, "task %d: element size must be 4 or 8", i);
+        IBP_CHECK(t.vec_size > 0 && t.n >= 0 && t.num_vecs >= 0, "task %d: invalid shape", i);
+        IBP_CHECK(t.index != nullptr || t.n <= t.num_vecs, "task %d: n exceeds num_vecs without an index", i);
+        IBP_CHECK(t.index == nullptr || t.index_bytes == 4 || t.index_bytes == 8, "task %d: index must be int32/int64", i);
+        IBP_CHECK(t.n == 0 || (t.comp && t.mask && t.bitval && t.bitmask && t.out), "task %d: null pointer", i);
+    }
+    ibp_plan *pl = new ibp_plan();
+    pl->tasks.assign(tasks, tasks + num_tasks);
+    (void)hipGetDevice(&pl->device);
+    const DeviceInfo &dev = device_info();
+    pl->cfg.blocks = dev.num_cus * 4;
+    if (hipMalloc((void **)&pl->dev_tasks, sizeof(TaskDev) * (size_t)num_tasks) != hipSuccess ||
+        hipMalloc((void **)&pl->head, sizeof(unsigned long long)) != hipSuccess) {
+        ibp_plan_destroy(pl);
+        return set_error(IBP_ERR_NOMEM, "plan allocation failed");
+    }
+    const int st = rebuild(pl);
+    if (st != IBP_OK) {
+        ibp_plan_destroy(pl);
+        return st;
+    }
+    *plan = pl;
+    return IBP_OK;
+}
+
+extern "C" int ibp_plan_destroy(ibp_plan *pl)
+{
+    if (!pl) return IBP_OK;
+    if (pl->graph_exec) (void)hipGraphExecDestroy(pl->graph_exec);
+    if (pl->graph) (void)hipGraphDestroy(pl->graph);
+    if (pl->capture_stream) (void)hipStreamDestroy(pl->capture_stream);
+    if (pl->dev_tasks) (void)hipFree(pl->dev_tasks);
+    if (pl->head) (void)hipFree(pl->head);
+    delete pl;
+    return IBP_OK;
+}
+
+extern "C" int ibp_plan_launch(ibp_plan *pl, void *stream)
+{
+    IBP_CHECK(pl != nullptr, "null plan");
+    return launch(pl, as_stream(stream));
+}
+
+extern "C" int ibp_plan_autotune(ibp_plan *pl, void *stream, float *best_ms)
+{
+    IBP_CHECK(pl != nullptr, "null plan");
+    hipStream_t s = as_stream(stream);
+    const DeviceInfo &dev = device_info();
+    std::vector<BatchConfig> cands;
+    for (int rpi : {16, 32, 64, 128})
+        for (int rif : {2, 4, 8})
+            for (int mult : {4, 8, 16}) cands.push_back(BatchConfig{rpi, rpi >= 64 ? 4 : 2, rif, dev.num_cus * mult});
+    hipEvent_t e0, e1;
+    IBP_HIP(hipEventCreate(&e0));
+    IBP_HIP(hipEventCreate(&e1));
+    float best = 1e30f;
+    BatchConfig best_cfg = pl->cfg;
+    for (const BatchConfig &c : cands) {
+        pl->cfg = c;
+        int st = rebuild(pl);
+        if (st != IBP_OK) return st;
+        st = launch(pl, s); // warm-up
+        if (st != IBP_OK) return st;
+        IBP_HIP(hipEventRecord(e0, s));
+        const int reps = 5;
+        for (int r = 0; r < reps; ++r)
+            if ((st = launch(pl, s)) != IBP_OK) return st;
+        IBP_HIP(hipEventRecord(e1, s));
+        IBP_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        IBP_HIP(hipEventElapsedTime(&ms, e0, e1));
+        ms /= reps;
+        if (ms < best) {
+            best = ms;
+            best_cfg = c;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    pl->cfg = best_cfg;
+    const int st = rebuild(pl);
+    if (st != IBP_OK) return st;
+    if (best_ms) *best_ms = best;
+    dprintf("[ibp] autotune: rows/item %d/%d, rows in flight %d, blocks %d: %.4f ms\n", best_cfg.rows_per_item_small,
+            best_cfg.rows_per_item_large, best_cfg.rows_in_flight, best_cfg.blocks, best);
+    return IBP_OK;
+}
+
+extern "C" int ibp_plan_capture_graph(ibp_plan *pl, void *stream)
+{
+    IBP_CHECK(pl != nullptr, "null plan");
+    (void)stream;
+    if (pl->graph_exec) return IBP_OK;
+    if (!pl->capture_stream) IBP_HIP(hipStreamCreateWithFlags(&pl->capture_stream, hipStreamNonBlocking));
+    IBP_HIP(hipStreamBeginCapture(pl->capture_stream, hipStreamCaptureModeThreadLocal));
+    const int st = launch(pl, pl->capture_stream);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(pl->capture_stream, &g);
+    if (st != IBP_OK) return st;
+    if (e != hipSuccess) return set_error(IBP_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+    pl->graph = g;
+    IBP_HIP(hipGraphInstantiate(&pl->graph_exec, pl->graph, nullptr, nullptr, 0));
+    return IBP_OK;
+}
+
+extern "C" int ibp_plan_graph_launch(ibp_plan *pl, void *stream)
+{
+    IBP_CHECK(pl != nullptr, "null plan");
+    if (!pl->graph_exec) {
+        const int st = ibp_plan_capture_graph(pl, stream);
+        if (st != IBP_OK) return st;
+    }
+    IBP_HIP(hipGraphLaunch(pl->graph_exec, as_stream(stream)));
+    return IBP_OK;
+}
+
+extern "C" int ibp_plan_describe(ibp_plan *pl, char *buf, size_t len)
+{
+    IBP_CHECK(pl != nullptr && buf != nullptr, "null argument");
+    snprintf(buf, len,
+             "{\"tasks\": %zu, \"items\": %lld, \"rows_per_item_small\": %d, \"rows_per_item_large\": %d, "
+             "\"rows_in_flight\": %d, \"blocks\": %d, \"graph\": %s}",
+             pl->tasks.size(), (long long)pl->total_items, pl->cfg.rows_per_item_small, pl->cfg.rows_per_item_large,
+             pl->cfg.rows_in_flight, pl->cfg.blocks, pl->graph_exec ? "true" : "false");
+    return IBP_OK;
+}

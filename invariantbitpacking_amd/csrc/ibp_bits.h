@@ -1,0 +1,106 @@
+// ibp_bits.h -- element-level bit arithmetic of the IBP slot codec.
+//
+// Compiled for the device (gfx950) and for the host (self-test).  T is
+// uint32_t or uint64_t (B = 32 or 64 bits); 1- and 2-byte elements only occur
+// in popcount/size work and use the 32-bit helpers on zero-extended values.
+//
+// Shift semantics: the reference relies on PTX `shl`/`shr` returning 0 for
+// counts >= B (ibp_compress_dev.cuh:47,53-55,80; ibp_decompress_dev.cuh:315,
+// 327-328).  gfx950 v_lshlrev_b64 uses the count modulo 64, so every shift
+// whose count can reach B goes through shl()/shr() below.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ibp {
+
+template <typename T>
+struct Bits {
+    static constexpr int B = 8 * (int)sizeof(T);
+};
+
+__host__ __device__ __forceinline__ int popc(uint32_t v) { return __builtin_popcount(v); }
+__host__ __device__ __forceinline__ int popc(uint64_t v) { return __builtin_popcountll(v); }
+__host__ __device__ __forceinline__ int clz(uint32_t v) { return v ? __builtin_clz(v) : 32; }
+__host__ __device__ __forceinline__ int clz(uint64_t v) { return v ? __builtin_clzll(v) : 64; }
+
+// v << s and v >> s with the PTX clamp: counts >= B give 0.
+template <typename T>
+__host__ __device__ __forceinline__ T shl(T v, int s)
+{
+    return s >= Bits<T>::B ? T(0) : T(v << s);
+}
+template <typename T>
+__host__ __device__ __forceinline__ T shr(T v, int s)
+{
+    return s >= Bits<T>::B ? T(0) : T(v >> s);
+}
+// Top n bits set (n in [0, B]).
+template <typename T>
+__host__ __device__ __forceinline__ T himask(int n)
+{
+    return n <= 0 ? T(0) : T(~T(0) << (Bits<T>::B - n));
+}
+
+// Gathers the bits of x at the set positions of `free` (the zero positions of
+// the column mask), highest first, into the top of the result; bits below the
+// gathered ones are zero.  This is the value compress_and_write builds in
+// `compressed_val` (ibp_compress_dev.cuh:39-58), one run of free bits per step.
+template <typename T>
+__host__ __device__ __forceinline__ T extract_msb(T x, T free)
+{
+    T out = 0;
+    int filled = 0;
+    while (free) {
+        const int s = clz(free);             // masked bits above this run
+        const int r = clz(T(~T(free << s))); // run length (free << s != 0, s < B)
+        const T run = shr(himask<T>(r), s);
+        out |= shr(shl(T(x & run), s), filled);
+        filled += r;
+        free &= ~run;
+    }
+    return out;
+}
+
+// Inverse: drops the top bits of v, highest first, into the set positions of
+// `free`, highest position first (the decode loop of decompress_fetch_cpu,
+// ibp_decompress_dev.cuh:309-334).  Bits of v beyond popc(free) are ignored.
+template <typename T>
+__host__ __device__ __forceinline__ T deposit_msb(T v, T free)
+{
+    T out = 0;
+    while (free) {
+        const int s = clz(free);
+        const int r = clz(T(~T(free << s)));
+        const T run = shr(himask<T>(r), s);
+        out |= shr(v, s) & run;
+        v = shl(v, r);
+        free &= ~run;
+    }
+    return out;
+}
+
+// The top B bits of the bit stream starting at bit o (0 <= o < B) of word w0,
+// continuing into w1: (w0 << o) | (w1 >> (B - o)).
+template <typename T>
+__host__ __device__ __forceinline__ T stream_bits(T w0, T w1, int o)
+{
+    return T(w0 << o) | shr(w1, Bits<T>::B - o);
+}
+
+// Header and slot geometry (ibp_compress_dev.cuh:13-15; ibp_misc_dev.cuh:33-42).
+__host__ __device__ __forceinline__ int64_t header_bytes(int64_t vec_size, int w)
+{
+    return ((vec_size + 7) / 8 + w - 1) / w * w;
+}
+
+// s(r) = min(V*w, align_w(ceil(V/8)) + align_w(V*w - floor(S/8))).
+__host__ __device__ __forceinline__ int64_t slot_size(int64_t vec_size, int w, int64_t saved_bits)
+{
+    const int64_t vec_bytes = vec_size * w;
+    const int64_t data = (vec_bytes - saved_bits / 8 + w - 1) / w * w;
+    const int64_t comp = header_bytes(vec_size, w) + data;
+    return comp < vec_bytes ? comp : vec_bytes;
+}
+
+} // namespace ibp

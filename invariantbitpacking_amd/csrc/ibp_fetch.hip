@@ -1,0 +1,174 @@
+// ibp_fetch.hip -- the plain per-call decompress_fetch.
+//
+// Replaces src/decompress.cu:10-156 and include/decompress/ibp_decompress_host.cuh:9-168.
+// Differences by design (values are identical):
+//  * no index sort and no temporary tensors: each wave gathers its own rows
+//    (the reference's CUB sort only orders the work, src/decompress.cu:64-98);
+//  * no zero-fill of the output: every element is written;
+//  * no per-call device queries: properties are cached per device;
+//  * out-of-range indices give zero rows and bump a device counter instead of
+//    reading out of bounds.
+#include "ibp_fetch_dev.h"
+#include "ibp_internal.h"
+
+namespace ibp {
+
+template <typename T, typename IdxT, int ROWS>
+__global__ __launch_bounds__(256) void fetch_small_kernel(FetchParams p)
+{
+    const SmallCols<T> col = load_small_cols<T>(p.mask, p.bitval, (int)p.vec_size);
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
+    for (int64_t base = wave * ROWS; base < p.n; base += nwaves * ROWS)
+        fetch_small_group<T, IdxT, ROWS>(p, col, base);
+}
+
+template <typename T, typename IdxT>
+__global__ __launch_bounds__(256) void fetch_large_kernel(FetchParams p)
+{
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
+    for (int64_t i = wave; i < p.n; i += nwaves) fetch_large_row<T, IdxT>(p, i);
+}
+
+// Transfer baselines of the reference (include/decompress/ibp_decompress_host.cuh:114-166):
+//  mode 2: raw gather of whole slots, no unpack (IndexSelectMultiKernelAligned);
+//  mode 3: packed rows read comp_len words and write slot[k % comp_len]
+//          (decompress_fake_kernel), raw rows copied;
+//  mode 4: packed rows read comp_len words and write nothing, raw rows
+//          untouched (decompress_fake_kernel2).
+template <typename T, typename IdxT>
+__global__ __launch_bounds__(256) void fetch_baseline_kernel(FetchParams p, int mode, int comp_len)
+{
+    const int lane = lane_id();
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
+    const int64_t V = p.vec_size;
+    for (int64_t i = wave; i < p.n; i += nwaves) {
+        const int64_t s = readfirstlane(load_index<IdxT>(p.index, i));
+        T *dst = (T *)p.out + i * V;
+        if (s < 0 || s >= p.num_vecs) {
+            for (int64_t k = lane; k < V; k += kWave) dst[k] = T(0);
+            continue;
+        }
+        const T *slot = (const T *)p.comp + s * V;
+        const bool packed = (p.bitmask[s >> 5] >> (s & 31)) & 1u;
+        if (mode == 2 || !packed) {
+            if (mode != 4)
+                for (int64_t k = lane; k < V; k += kWave) dst[k] = slot[k];
+            continue;
+        }
+        const int64_t cl = comp_len < V ? (comp_len > 0 ? comp_len : 1) : V;
+        if (mode == 3 && cl <= kWave) {
+            // One read of the first cl words, replicated through the row.
+            const T x = lane < cl ? slot[lane] : T(0);
+            for (int64_t k0 = 0; k0 < V; k0 += kWave) {
+                const int64_t k = k0 + lane;
+                const T y = shfl(x, (int)(k % cl));
+                if (k < V) dst[k] = y;
+            }
+        } else if (mode == 3) {
+            for (int64_t k = lane; k < V; k += kWave) dst[k] = slot[k % cl];
+        } else {
+            T acc = T(0);
+            for (int64_t k = lane; k < cl; k += kWave) acc ^= slot[k];
+            asm volatile("" : : "v"(acc)); // the reads are the point of this mode
+        }
+    }
+}
+
+namespace {
+
+int pick_rows(int64_t n)
+{
+    (void)n;
+    return 4;
+}
+
+template <typename T, typename IdxT>
+int launch_typed(const FetchParams &p, int impl, int comp_len, int grid, int block, hipStream_t stream)
+{
+    if (impl == 2 || impl == 3 || impl == 4) {
+        hipLaunchKernelGGL((fetch_baseline_kernel<T, IdxT>), dim3(grid), dim3(block), 0, stream, p, impl, comp_len);
+    } else if (p.vec_size <= kWave) {
+        switch (pick_rows(p.n)) {
+        case 8: hipLaunchKernelGGL((fetch_small_kernel<T, IdxT, 8>), dim3(grid), dim3(block), 0, stream, p); break;
+        case 2: hipLaunchKernelGGL((fetch_small_kernel<T, IdxT, 2>), dim3(grid), dim3(block), 0, stream, p); break;
+        default: hipLaunchKernelGGL((fetch_small_kernel<T, IdxT, 4>), dim3(grid), dim3(block), 0, stream, p); break;
+        }
+    } else {
+        hipLaunchKernelGGL((fetch_large_kernel<T, IdxT>), dim3(grid), dim3(block), 0, stream, p);
+    }
+    IBP_LAUNCH_CHECK();
+    return IBP_OK;
+}
+
+} // namespace
+
+int fetch_launch(const FetchParams &p, int elem_bytes, int index_bytes, int impl, int comp_len, int nblks,
+                 int nthreads, hipStream_t stream)
+{
+    const DeviceInfo &dev = device_info();
+    // Blocks are at most 256 threads (four waves); larger requests are capped.
+    const int block = (nthreads > 0 && nthreads <= 256) ? nthreads : 256;
+    int64_t waves_needed = p.vec_size <= kWave ? (p.n + 3) / 4 : p.n;
+    int64_t grid = nblks > 0 ? nblks : (waves_needed * kWave + block - 1) / block;
+    const int64_t cap = (int64_t)dev.num_cus * 8;
+    if (nblks <= 0 && grid > cap) grid = cap;
+    if (grid < 1) grid = 1;
+    const bool idx64 = index_bytes == 8;
+    if (elem_bytes == 8) {
+        if (p.index == nullptr || idx64) return launch_typed<uint64_t, int64_t>(p, impl, comp_len, (int)grid, block, stream);
+        return launch_typed<uint64_t, int32_t>(p, impl, comp_len, (int)grid, block, stream);
+    }
+    if (p.index == nullptr || idx64) return launch_typed<uint32_t, int64_t>(p, impl, comp_len, (int)grid, block, stream);
+    return launch_typed<uint32_t, int32_t>(p, impl, comp_len, (int)grid, block, stream);
+}
+
+// Words of the speculative first read of a packed small row: the expected
+// compressed length plus one word of slack, at least the header.
+int first_words_for(int64_t vec_size, int elem_bytes, int comp_len)
+{
+    const int64_t hw = header_bytes(vec_size, elem_bytes) / elem_bytes;
+    int64_t e = comp_len > 0 ? (int64_t)comp_len + 1 : vec_size;
+    if (e < hw + 1) e = hw + 1;
+    if (e > vec_size) e = vec_size;
+    return (int)e;
+}
+
+} // namespace ibp
+
+using namespace ibp;
+
+extern "C" int ibp_decompress_fetch(int elem_bytes, void *out, const void *comp, int64_t num_vecs,
+                                    int64_t vec_size, const void *mask, const void *bitval,
+                                    const int32_t *bitmask, const void *index, int index_bytes, int64_t n,
+                                    int comp_len, int nblks, int nthreads, int impl, int comp_loc, void *stream)
+{
+    (void)comp_loc;
+    IBP_CHECK(elem_bytes == 4 || elem_bytes == 8, "Input tensor must be 4-byte or 8-byte datatype (got %d)", elem_bytes);
+    IBP_CHECK(vec_size > 0 && num_vecs >= 0 && n >= 0, "invalid shape num_vecs=%lld vec_size=%lld n=%lld",
+              (long long)num_vecs, (long long)vec_size, (long long)n);
+    IBP_CHECK(index == nullptr || index_bytes == 4 || index_bytes == 8, "index must be int32 or int64");
+    IBP_CHECK(impl >= -1 && impl <= 5, "unknown impl %d", impl);
+    IBP_CHECK(nthreads <= 0 || (nthreads % 64 == 0 && nthreads <= 1024), "nthreads must be a multiple of 64 up to 1024");
+    if (n == 0) return IBP_OK;
+    IBP_CHECK(out && comp && mask && bitval && bitmask, "null pointer argument");
+    IBP_CHECK(index != nullptr || n <= num_vecs, "without an index n (%lld) must not exceed num_vecs (%lld)",
+              (long long)n, (long long)num_vecs);
+    FetchParams p;
+    p.out = out;
+    p.comp = comp;
+    p.mask = mask;
+    p.bitval = bitval;
+    p.bitmask = (const uint32_t *)bitmask;
+    p.index = index;
+    p.num_vecs = num_vecs;
+    p.vec_size = vec_size;
+    p.n = n;
+    p.first_words = first_words_for(vec_size, elem_bytes, comp_len);
+    p.pad = 0;
+    p.err = index ? index_error_counter() : nullptr;
+    const int cl = comp_len > 0 ? comp_len : (int)vec_size;
+    return fetch_launch(p, elem_bytes, index_bytes, impl, cl, nblks, nthreads, as_stream(stream));
+}

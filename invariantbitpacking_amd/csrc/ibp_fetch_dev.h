@@ -1,0 +1,259 @@
+// ibp_fetch_dev.h -- device routines that fetch and unpack rows of the IBP
+// slot format.  Shared by the plain fetch kernels (ibp_fetch.hip) and the
+// persistent batched kernel (ibp_batch.hip).
+//
+// Slot format (reference: ibp_compress_dev.cuh:13-15,61-92; decode
+// ibp_decompress_dev.cuh:279-336): V words of T.  A packed slot starts with
+// Hw = align_w(ceil(V/8))/w header words whose bit j (LSB first, word j/B)
+// flags element j; the payload that follows is an MSB-first bit stream of
+// B - popc(mask[j]) free bits per flagged element and B raw bits otherwise;
+// the rest of the slot is zero.  A row whose bitmask bit is clear is stored raw.
+//
+// Mapping: one wave64 per row.  Lane j owns element j of each 64-element chunk.
+// For w = 8 the 64-bit header word c holds exactly the flags of chunk c.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ibp_bits.h"
+#include "ibp_wave.h"
+
+namespace ibp {
+
+struct FetchParams {
+    void *out;
+    const void *comp;
+    const void *mask;
+    const void *bitval;
+    const uint32_t *bitmask;
+    const void *index; // nullptr: identity
+    int64_t num_vecs;
+    int64_t vec_size;
+    int64_t n;
+    int32_t first_words; // words read before the header is known (small rows)
+    int32_t pad;
+    unsigned long long *err;
+};
+
+template <typename IdxT>
+__device__ __forceinline__ int64_t load_index(const void *index, int64_t i)
+{
+    return index ? (int64_t)((const IdxT *)index)[i] : i;
+}
+
+// Column constants of lane j for rows of at most 64 elements.
+template <typename T>
+struct SmallCols {
+    T bitval;
+    T free; // ~mask: positions carried in the payload of a flagged element
+    int len_flagged; // B - popc(mask)
+    bool valid;      // j < V
+};
+
+template <typename T>
+__device__ __forceinline__ SmallCols<T> load_small_cols(const void *mask, const void *bitval, int V)
+{
+    const int lane = lane_id();
+    SmallCols<T> c;
+    c.valid = lane < V;
+    const T m = c.valid ? ((const T *)mask)[lane] : T(0);
+    c.bitval = c.valid ? ((const T *)bitval)[lane] : T(0);
+    c.free = T(~m);
+    c.len_flagged = Bits<T>::B - popc(m);
+    return c;
+}
+
+// Header flag of lane `lane` for a row of <= 64 elements, given the first
+// slot words held one per lane in `w`.
+template <typename T>
+__device__ __forceinline__ bool small_flag(T w, int lane)
+{
+    if constexpr (sizeof(T) == 8) {
+        const uint64_t h = readlane((uint64_t)w, 0);
+        return (h >> lane) & 1ull;
+    } else {
+        const uint32_t h0 = readlane((uint32_t)w, 0);
+        const uint32_t h1 = readlane((uint32_t)w, 1);
+        return ((lane < 32 ? h0 : h1) >> (lane & 31)) & 1u;
+    }
+}
+
+// Fetches and unpacks ROWS consecutive output rows [base, base + ROWS) of a
+// table whose rows have at most 64 elements.  All loads of one stage are
+// issued for every row before any of them is consumed.
+template <typename T, typename IdxT, int ROWS>
+__device__ __forceinline__ void fetch_small_group(const FetchParams &p, const SmallCols<T> &col, int64_t base)
+{
+    constexpr int B = Bits<T>::B;
+    const int lane = lane_id();
+    const int V = (int)p.vec_size;
+    const int Hw = (int)(header_bytes(V, (int)sizeof(T)) / (int)sizeof(T));
+    const T *comp = (const T *)p.comp;
+    T *out = (T *)p.out;
+
+    // Stage 1: indices of the group (one load), then bitmask words (one gather).
+    int64_t my_src = 0;
+    if (lane < ROWS && base + lane < p.n) my_src = load_index<IdxT>(p.index, base + lane);
+    const bool my_ok = lane < ROWS && base + lane < p.n && my_src >= 0 && my_src < p.num_vecs;
+    uint32_t my_bm = 0;
+    if (my_ok) my_bm = p.bitmask[my_src >> 5];
+    const uint64_t okmask = ballot(my_ok);
+    const uint64_t pkmask = ballot(my_ok && ((my_bm >> (my_src & 31)) & 1u));
+
+    int64_t src[ROWS];
+    int need[ROWS];
+    T w[ROWS];
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+        src[r] = readlane(my_src, r);
+        const bool ok = (okmask >> r) & 1ull;
+        const bool packed = (pkmask >> r) & 1ull;
+        need[r] = ok ? (packed ? p.first_words : V) : 0;
+    }
+    // Stage 2: first read of every row (whole raw rows, speculative prefix of
+    // packed ones).
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+        w[r] = T(0);
+        if (lane < need[r]) w[r] = comp[src[r] * V + lane];
+    }
+    // Stage 3: header -> flags -> bit offsets; second read where the prefix was short.
+    int pos[ROWS];
+    bool flag[ROWS];
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+        const bool packed = (pkmask >> r) & 1ull;
+        flag[r] = false;
+        pos[r] = 0;
+        if (packed) { // wave-uniform
+            flag[r] = col.valid && small_flag<T>(w[r], lane);
+            const int L = col.valid ? (flag[r] ? col.len_flagged : B) : 0;
+            const int incl = wave_incl_scan(L);
+            const int total = readlane(incl, kWave - 1);
+            pos[r] = incl - L;
+            // Clamped to the slot: a corrupt header cannot make the read leave the row.
+            const int nw = Hw + (total + B - 1) / B;
+            need[r] = nw < V ? nw : V;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+        const bool packed = (pkmask >> r) & 1ull;
+        if (packed && need[r] > p.first_words && lane >= p.first_words && lane < need[r])
+            w[r] = comp[src[r] * V + lane];
+    }
+    // Stage 4: unpack and store.
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+        const int64_t i = base + r;
+        const bool ok = (okmask >> r) & 1ull;
+        const bool packed = (pkmask >> r) & 1ull;
+        T val = w[r];
+        if (packed) {
+            const int q = Hw + pos[r] / B;
+            const int o = pos[r] % B;
+            const T w0 = shfl(w[r], q);
+            const T w1 = shfl(w[r], q + 1 < kWave ? q + 1 : kWave - 1);
+            const T v = stream_bits<T>(w0, w1, o);
+            val = flag[r] ? T(col.bitval | deposit_msb<T>(v, col.free)) : v;
+        }
+        if (i < p.n && lane < V) out[i * V + lane] = ok ? val : T(0);
+    }
+    // The ballot needs the whole wave active, so it is taken before the lane-0 branch.
+    const uint64_t live = ballot(lane < ROWS && base + lane < p.n);
+    if (p.err && lane == 0) {
+        const int bad = __builtin_popcountll(live & ~okmask);
+        if (bad) atomicAdd(p.err, (unsigned long long)bad);
+    }
+}
+
+// Fetches and unpacks output row i of a table with rows of any length, one
+// 64-element chunk at a time.  Wave-uniform.
+template <typename T, typename IdxT>
+__device__ __forceinline__ void fetch_large_row(const FetchParams &p, int64_t i)
+{
+    constexpr int B = Bits<T>::B;
+    constexpr int FLAGS_PER_WORD = B;
+    const int lane = lane_id();
+    const int64_t V = p.vec_size;
+    const int64_t Hw = header_bytes(V, (int)sizeof(T)) / (int64_t)sizeof(T);
+    const T *mask = (const T *)p.mask;
+    const T *bitval = (const T *)p.bitval;
+    T *dst = (T *)p.out + i * V;
+
+    const int64_t s = readfirstlane(load_index<IdxT>(p.index, i));
+    if (s < 0 || s >= p.num_vecs) {
+        for (int64_t k = lane; k < V; k += kWave) dst[k] = T(0);
+        if (p.err && lane == 0) atomicAdd(p.err, 1ull);
+        return;
+    }
+    const T *slot = (const T *)p.comp + s * V;
+    const bool packed = (p.bitmask[s >> 5] >> (s & 31)) & 1u;
+    if (!packed) {
+        // Raw row: four independent loads in flight per lane.
+        for (int64_t k0 = lane; k0 < V; k0 += 4 * kWave) {
+            T a[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int64_t k = k0 + u * kWave;
+                a[u] = k < V ? slot[k] : T(0);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int64_t k = k0 + u * kWave;
+                if (k < V) dst[k] = a[u];
+            }
+        }
+        return;
+    }
+    const T *payload = slot + Hw;
+    int64_t hdr_group = -1;
+    T hdr = T(0);
+    int64_t cursor = 0; // bit position in the payload
+    const int64_t nchunks = (V + kWave - 1) / kWave;
+    for (int64_t c = 0; c < nchunks; ++c) {
+        const int64_t j = c * kWave + lane;
+        const bool valid = j < V;
+        const T m = valid ? mask[j] : T(0);
+        const T b = valid ? bitval[j] : T(0);
+        // Header words covering this chunk: word (c*64 + lane) / B.
+        const int64_t wlo = (c * kWave) / FLAGS_PER_WORD;
+        const int64_t whi = (c * kWave + kWave - 1) / FLAGS_PER_WORD;
+        if (whi / kWave != hdr_group) { // header words are staged 64 at a time
+            hdr_group = whi / kWave;
+            const int64_t hk = hdr_group * kWave + lane;
+            hdr = hk < Hw ? slot[hk] : T(0);
+        }
+        T hw;
+        if (wlo == whi) {
+            hw = readlane(hdr, (int)(wlo % kWave));
+        } else { // w = 4: two header words per chunk
+            const T h0 = readlane(hdr, (int)(wlo % kWave));
+            const T h1 = readlane(hdr, (int)(whi % kWave));
+            hw = lane < FLAGS_PER_WORD ? h0 : h1;
+        }
+        const bool f = valid && ((hw >> (j % FLAGS_PER_WORD)) & 1);
+        const int L = valid ? (f ? B - popc(m) : B) : 0;
+        const int incl = wave_incl_scan(L);
+        const int tot = readlane(incl, kWave - 1);
+        const int64_t p_bit = cursor + incl - L;
+        const int64_t w0 = cursor / B;
+        const int64_t cnt = (cursor + tot + B - 1) / B - w0; // <= 65
+        // Reads stay inside the slot even for a corrupt header.
+        const int64_t room = V - Hw - w0;
+        const T A = (lane < cnt && lane < room) ? payload[w0 + lane] : T(0);
+        const T X = (cnt > kWave && kWave < room) ? payload[w0 + kWave] : T(0);
+        const int q = (int)(p_bit / B - w0);
+        const int o = (int)(p_bit % B);
+        const T s0 = shfl(A, q & (kWave - 1));
+        const T s1 = shfl(A, (q + 1) & (kWave - 1));
+        const T a0 = q < kWave ? s0 : X;
+        const T a1 = q + 1 < kWave ? s1 : X;
+        const T v = stream_bits<T>(a0, a1, o);
+        const T val = f ? T(b | deposit_msb<T>(v, T(~m))) : v;
+        if (valid) dst[j] = val;
+        cursor += tot;
+    }
+}
+
+} // namespace ibp

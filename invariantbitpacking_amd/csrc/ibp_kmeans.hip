@@ -1,0 +1,310 @@
+// ibp_kmeans.hip -- bitwise k-means mask selection (preprocess_kmeans).
+//
+// Replaces src/preprocess.cu:88-139 and ibp::preproc_kmeans
+// (include/preproc/ibp_preproc_host.cuh:119-284) with its kernels
+// cluster_vecs, compute_new_centroids, construct_bitmasks and create_mask_many
+// (include/preproc/ibp_preproc_kmeans.cuh:106-297).
+//
+// Semantics kept from the reference:
+//  * centroid 0 = row 0; centroids 1..min(K,N)-1 = rows at positions 1.. of a
+//    libstdc++-style random_shuffle of 0..N-1 driven by rand();
+//  * 10 Lloyd iterations of Hamming-distance assignment and majority-bit
+//    centroids, where a 32-lane accumulator is never cleared between the
+//    columns l, l+32, l+64, ... it visits (ibp_preproc_kmeans.cuh:226-233);
+//  * the returned masks are those of the last sweep threshold (1.0f): a bit is
+//    in mask c iff it is constant over cluster c; an empty cluster gets all ones;
+//  * the threshold argument is ignored.
+// Defined here where the reference is undefined: ties in the assignment go to
+// the highest centroid index (the reference races on atomicExch); for 8-byte
+// elements every thread owns 64 counters (the reference overruns its
+// 8192-float shared array); centroids beyond N rows are zero.
+#include <stdlib.h>
+
+#include <vector>
+
+#include "ibp_bits.h"
+#include "ibp_internal.h"
+#include "ibp_wave.h"
+
+namespace ibp {
+
+template <typename T>
+using KW = typename std::conditional<sizeof(T) == 8, uint64_t, uint32_t>::type;
+
+template <typename T>
+__global__ void km_gather_kernel(const T *data, int64_t V, const int64_t *rows, int K, T *centroids)
+{
+    const int64_t total = (int64_t)K * V;
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c = g / V, j = g % V;
+        const int64_t r = rows[c];
+        centroids[g] = r >= 0 ? data[r * V + j] : T(0);
+    }
+}
+
+// Nearest centroid by popc(x ^ c); equal distances go to the highest index.
+template <typename T>
+__global__ __launch_bounds__(256) void km_assign_kernel(const T *data, int64_t N, int64_t V, const T *centroids,
+                                                        int K, int32_t *cluster)
+{
+    const int lane = lane_id();
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
+    for (int64_t i = wave; i < N; i += nwaves) {
+        int best = 0;
+        int best_d = 0x7fffffff;
+        for (int c = 0; c < K; ++c) {
+            int d = 0;
+            for (int64_t j = lane; j < V; j += kWave) d += popc((KW<T>)(data[i * V + j] ^ centroids[(int64_t)c * V + j]));
+            d = wave_sum(d);
+            if (d <= best_d) {
+                best_d = d;
+                best = c;
+            }
+        }
+        if (lane == 0) cluster[i] = best;
+    }
+}
+
+__global__ void km_hist_kernel(const int32_t *cluster, int64_t N, int32_t *count)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x)
+        atomicAdd(&count[cluster[i]], 1);
+}
+
+__global__ void km_offsets_kernel(const int32_t *count, int K, int32_t *offset, int32_t *fill)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    int32_t acc = 0;
+    for (int c = 0; c < K; ++c) {
+        offset[c] = acc;
+        fill[c] = 0;
+        acc += count[c];
+    }
+}
+
+__global__ void km_scatter_kernel(const int32_t *cluster, int64_t N, const int32_t *offset, int32_t *fill,
+                                  int32_t *perm)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+        const int c = cluster[i];
+        perm[offset[c] + atomicAdd(&fill[c], 1)] = (int32_t)i;
+    }
+}
+
+// bits_set[(c*V + j)*B + k] = members of cluster c whose column j has bit k set.
+// grid = (column tiles, K, row splits); lane <-> column, waves split the rows.
+template <typename T>
+__global__ __launch_bounds__(256) void km_bitcount_kernel(const T *data, int64_t V, const int32_t *perm,
+                                                          const int32_t *offset, const int32_t *count,
+                                                          int32_t *bits_set)
+{
+    constexpr int B = 8 * (int)sizeof(T);
+    const int lane = lane_id();
+    const int c = blockIdx.y;
+    const int64_t j = (int64_t)blockIdx.x * kWave + lane;
+    if (j >= V) return;
+    const int waves = blockDim.x / kWave;
+    const int64_t stride = (int64_t)waves * gridDim.z;
+    const int32_t *rows = perm + offset[c];
+    const int64_t nrows = count[c];
+    int32_t acc[B];
+#pragma unroll
+    for (int k = 0; k < B; ++k) acc[k] = 0;
+    for (int64_t q = (int64_t)blockIdx.z * waves + threadIdx.x / kWave; q < nrows; q += stride) {
+        const KW<T> x = (KW<T>)data[(int64_t)rows[q] * V + j];
+#pragma unroll
+        for (int k = 0; k < B; ++k) acc[k] += (int32_t)((x >> k) & 1);
+    }
+    int32_t *dst = bits_set + ((int64_t)c * V + j) * B;
+#pragma unroll
+    for (int k = 0; k < B; ++k)
+        if (acc[k]) atomicAdd(&dst[k], acc[k]);
+}
+
+// construct_bitmasks (ibp_preproc_kmeans.cuh:219-236): thread (c, l), l < 32,
+// walks columns l, l+32, ... and ORs majority bits into one accumulator.
+template <typename T>
+__global__ void km_construct_kernel(int64_t V, int K, const int32_t *count, const int32_t *bits_set, T *centroids)
+{
+    constexpr int B = 8 * (int)sizeof(T);
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= K * 32) return;
+    const int c = g / 32, l = g % 32;
+    const int32_t half = count[c] / 2;
+    uint64_t acc = 0;
+    for (int64_t id = l; id < V; id += 32) {
+        const int32_t *bs = bits_set + ((int64_t)c * V + id) * B;
+        for (int k = 0; k < B; ++k)
+            if (bs[k] > half) acc |= 1ull << k;
+        centroids[(int64_t)c * V + id] = (T)acc;
+    }
+}
+
+// create_mask_many at threshold t (ibp_preproc_kmeans.cuh:273-293):
+// float(bits) >= t * float(members) -> mask and bitval;
+// double(bits) <= (1.0 - t) * members -> mask only.
+template <typename T>
+__global__ void km_masks_kernel(int64_t V, int K, float t, const int32_t *count, const int32_t *bits_set, T *masks,
+                                T *bitvals)
+{
+    constexpr int B = 8 * (int)sizeof(T);
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (int64_t)K * V) return;
+    const int c = (int)(g / V);
+    const float hi = __fmul_rn(t, (float)count[c]);
+    const double lo = __dmul_rn(1.0 - (double)t, (double)count[c]);
+    const int32_t *bs = bits_set + g * B;
+    uint64_t val = 0, m = 0;
+    for (int k = 0; k < B; ++k) {
+        const float f = (float)bs[k];
+        if (f >= hi) {
+            val |= 1ull << k;
+            m |= 1ull << k;
+        } else if ((double)f <= lo) {
+            m |= 1ull << k;
+        }
+    }
+    masks[g] = (T)m;
+    bitvals[g] = (T)val;
+}
+
+__global__ void km_zero_kernel(int32_t *p, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
+float sweep_last_threshold()
+{
+    float last = 0.f;
+    for (volatile float t = 0.7f; t <= 1.0f; t = (float)((double)t + 0.05)) last = t;
+    return last;
+}
+
+// libstdc++ random_shuffle(first, last) over 0..n-1 with rand().
+void libstdcxx_random_shuffle(std::vector<int32_t> &a)
+{
+    const int64_t n = (int64_t)a.size();
+    for (int64_t i = 1; i < n; ++i) {
+        const int64_t j = (int64_t)(rand() % (int)(i + 1));
+        if (i != j) std::swap(a[i], a[j]);
+    }
+}
+
+namespace {
+
+unsigned blocks_for(int64_t n, int64_t cap = 4096)
+{
+    int64_t b = (n + 255) / 256;
+    if (b < 1) b = 1;
+    if (b > cap) b = cap;
+    return (unsigned)b;
+}
+
+template <typename T>
+int kmeans_typed(const void *data_v, int64_t N, int64_t V, int K, int64_t seed, void *masks, void *bitvals,
+                 int32_t *clusters_out, hipStream_t s)
+{
+    constexpr int B = 8 * (int)sizeof(T);
+    const DeviceInfo &dev = device_info();
+    const T *data = (const T *)data_v;
+    // Host side initialisation (ibp_preproc_host.cuh:150-151,176-188).
+    std::vector<int64_t> init((size_t)K, -1);
+    if (N > 0) init[0] = 0;
+    if (seed >= 0) srand((unsigned)seed);
+    {
+        std::vector<int32_t> idx((size_t)N);
+        for (int64_t i = 0; i < N; ++i) idx[(size_t)i] = (int32_t)i;
+        libstdcxx_random_shuffle(idx);
+        for (int64_t c = 1; c < (K < N ? K : N); ++c) init[(size_t)c] = idx[(size_t)c];
+    }
+    T *cent = nullptr;
+    int32_t *cluster = nullptr, *count = nullptr, *offset = nullptr, *fill = nullptr, *perm = nullptr, *bits = nullptr;
+    int64_t *rows = nullptr;
+    const int64_t nbits = (int64_t)K * V * B;
+    auto cleanup = [&]() {
+        (void)hipFreeAsync(cent, s);
+        (void)hipFreeAsync(cluster, s);
+        (void)hipFreeAsync(count, s);
+        (void)hipFreeAsync(perm, s);
+        (void)hipFreeAsync(bits, s);
+        (void)hipFreeAsync(rows, s);
+    };
+    hipError_t e = hipSuccess;
+    if ((e = hipMallocAsync((void **)&cent, sizeof(T) * (size_t)(K * V), s)) != hipSuccess ||
+        (e = hipMallocAsync((void **)&cluster, sizeof(int32_t) * (size_t)(N > 0 ? N : 1), s)) != hipSuccess ||
+        (e = hipMallocAsync((void **)&count, sizeof(int32_t) * 3 * (size_t)K, s)) != hipSuccess ||
+        (e = hipMallocAsync((void **)&perm, sizeof(int32_t) * (size_t)(N > 0 ? N : 1), s)) != hipSuccess ||
+        (e = hipMallocAsync((void **)&bits, sizeof(int32_t) * (size_t)nbits, s)) != hipSuccess ||
+        (e = hipMallocAsync((void **)&rows, sizeof(int64_t) * (size_t)K, s)) != hipSuccess) {
+        cleanup();
+        return set_error(IBP_ERR_NOMEM, "kmeans workspace allocation failed: %s", hipGetErrorString(e));
+    }
+    offset = count + K;
+    fill = count + 2 * K;
+    IBP_HIP(hipMemcpyAsync(rows, init.data(), sizeof(int64_t) * (size_t)K, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL((km_gather_kernel<T>), dim3(blocks_for((int64_t)K * V)), dim3(256), 0, s, data, V, rows, K, cent);
+    hipLaunchKernelGGL(km_zero_kernel, dim3(blocks_for(N)), dim3(256), 0, s, cluster, N);
+    IBP_LAUNCH_CHECK();
+
+    int64_t assign_grid = (N * kWave + 255) / 256;
+    if (assign_grid > (int64_t)dev.num_cus * 8) assign_grid = (int64_t)dev.num_cus * 8;
+    if (assign_grid < 1) assign_grid = 1;
+    const unsigned tiles = (unsigned)((V + kWave - 1) / kWave);
+    unsigned splits = (unsigned)((N / (K > 0 ? K : 1)) / 1024 + 1);
+    if (splits > 64) splits = 64;
+    for (int it = 0; it < 10; ++it) {
+        hipLaunchKernelGGL((km_assign_kernel<T>), dim3((unsigned)assign_grid), dim3(256), 0, s, data, N, V, cent, K,
+                           cluster);
+        hipLaunchKernelGGL(km_zero_kernel, dim3(1), dim3(256), 0, s, count, (int64_t)K);
+        hipLaunchKernelGGL(km_zero_kernel, dim3(blocks_for(nbits)), dim3(256), 0, s, bits, nbits);
+        hipLaunchKernelGGL(km_hist_kernel, dim3(blocks_for(N)), dim3(256), 0, s, cluster, N, count);
+        hipLaunchKernelGGL(km_offsets_kernel, dim3(1), dim3(64), 0, s, count, K, offset, fill);
+        hipLaunchKernelGGL(km_scatter_kernel, dim3(blocks_for(N)), dim3(256), 0, s, cluster, N, offset, fill, perm);
+        hipLaunchKernelGGL((km_bitcount_kernel<T>), dim3(tiles, (unsigned)K, splits), dim3(256), 0, s, data, V, perm,
+                           offset, count, bits);
+        hipLaunchKernelGGL((km_construct_kernel<T>), dim3((unsigned)((K * 32 + 255) / 256)), dim3(256), 0, s, V, K,
+                           count, bits, cent);
+        e = hipGetLastError();
+        if (e != hipSuccess) {
+            cleanup();
+            return set_error(IBP_ERR_HIP, "kmeans iteration launch failed: %s", hipGetErrorString(e));
+        }
+    }
+    // Final masks from the last assignment's counts (still in count/bits).
+    hipLaunchKernelGGL((km_masks_kernel<T>), dim3(blocks_for((int64_t)K * V, 1 << 20)), dim3(256), 0, s, V, K,
+                       sweep_last_threshold(), count, bits, (T *)masks, (T *)bitvals);
+    e = hipGetLastError();
+    if (e == hipSuccess && N > 0)
+        e = hipMemcpyAsync(clusters_out, cluster, sizeof(int32_t) * (size_t)N, hipMemcpyDefault, s);
+    cleanup();
+    if (e != hipSuccess) return set_error(IBP_ERR_HIP, "kmeans finish failed: %s", hipGetErrorString(e));
+    IBP_HIP(hipStreamSynchronize(s));
+    return IBP_OK;
+}
+
+} // namespace
+
+} // namespace ibp
+
+using namespace ibp;
+
+extern "C" int ibp_preprocess_kmeans(int elem_bytes, const void *data, int64_t num_vecs, int64_t vec_size,
+                                     int num_clusters, float threshold, int64_t seed, void *masks, void *bitvals,
+                                     int32_t *clusters, void *stream)
+{
+    (void)threshold; // unused by the reference as well (ibp_preproc_host.cuh:119-284)
+    IBP_CHECK(elem_bytes == 4 || elem_bytes == 8, "Input tensor must be 4-byte or 8-byte datatype (got %d)", elem_bytes);
+    IBP_CHECK(vec_size > 0 && num_vecs >= 0, "invalid shape");
+    IBP_CHECK(num_clusters >= 1 && num_clusters <= 65535, "num_clusters must be in [1, 65535]");
+    IBP_CHECK(num_vecs < (int64_t)1 << 31, "num_vecs must fit int32 cluster ids and counters");
+    IBP_CHECK(masks && bitvals && clusters && (data || num_vecs == 0), "null pointer argument");
+    const int st = elem_bytes == 8
+                       ? kmeans_typed<uint64_t>(data, num_vecs, vec_size, num_clusters, seed, masks, bitvals, clusters,
+                                                as_stream(stream))
+                       : kmeans_typed<uint32_t>(data, num_vecs, vec_size, num_clusters, seed, masks, bitvals, clusters,
+                                                as_stream(stream));
+    if (st == IBP_OK) dprintf("[ibp] kmeans: K=%d V=%lld N=%lld\n", num_clusters, (long long)vec_size, (long long)num_vecs);
+    return st;
+}

@@ -1,0 +1,415 @@
+// ibp_serve.hip -- continuous-batching fetch server.
+//
+// No reference counterpart (the reference has no serving loop; its callers
+// issue decompress_fetch per request).  A request = (registered tensor set,
+// host index array).  Requests are admitted when their output fits the
+// device arena (admission by free HBM), merged with the other admitted
+// requests into ONE persistent-kernel launch, and launched on one of
+// `num_streams` streams as soon as a stream is free, so requests that arrive
+// while others run are batched and overlap on the device.  Each request's
+// output region and index range are its own, so a result never depends on
+// which other requests shared its launch.
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "ibp_batch.h"
+#include "ibp_internal.h"
+
+using namespace ibp;
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+double now_us()
+{
+    return std::chrono::duration<double, std::micro>(Clock::now().time_since_epoch()).count();
+}
+
+struct Set {
+    int elem_bytes;
+    const void *comp;
+    int64_t num_vecs, vec_size;
+    const void *mask, *bitval;
+    const int32_t *bitmask;
+    int comp_len, comp_loc;
+};
+
+enum class St { Pending, Admitted, Running, Done };
+
+struct Req {
+    int set = 0;
+    std::vector<int64_t> index;
+    int64_t n = 0;
+    size_t out_off = 0, out_bytes = 0;
+    double t_arrival = 0, t_done = 0;
+    St st = St::Pending;
+    bool waited = false;
+};
+
+// First-fit allocator over [0, size) with coalescing; 256-byte granules.
+struct Arena {
+    size_t size = 0, used = 0, peak = 0;
+    std::map<size_t, size_t> free_list; // offset -> length
+    void init(size_t s)
+    {
+        size = s;
+        free_list.clear();
+        free_list[0] = s;
+    }
+    bool alloc(size_t bytes, size_t cap, size_t *off)
+    {
+        bytes = (bytes + 255) / 256 * 256;
+        if (bytes == 0) bytes = 256;
+        if (used + bytes > cap) return false;
+        for (auto it = free_list.begin(); it != free_list.end(); ++it) {
+            if (it->second >= bytes) {
+                *off = it->first;
+                const size_t rest = it->second - bytes;
+                const size_t roff = it->first + bytes;
+                free_list.erase(it);
+                if (rest) free_list[roff] = rest;
+                used += bytes;
+                peak = std::max(peak, used);
+                return true;
+            }
+        }
+        return false;
+    }
+    void release(size_t off, size_t bytes)
+    {
+        bytes = (bytes + 255) / 256 * 256;
+        if (bytes == 0) bytes = 256;
+        used -= bytes;
+        auto it = free_list.emplace(off, bytes).first;
+        auto next = std::next(it);
+        if (next != free_list.end() && it->first + it->second == next->first) {
+            it->second += next->second;
+            free_list.erase(next);
+        }
+        if (it != free_list.begin()) {
+            auto prev = std::prev(it);
+            if (prev->first + prev->second == it->first) {
+                prev->second += it->second;
+                free_list.erase(it);
+            }
+        }
+    }
+};
+
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    TaskDev *dev_tasks = nullptr, *host_tasks = nullptr; // host: pinned staging
+    int64_t *dev_index = nullptr, *host_index = nullptr;
+    unsigned long long *head = nullptr;
+    std::vector<int64_t> reqs;
+    bool busy = false;
+};
+
+} // namespace
+
+struct ibp_server {
+    ibp_server_config cfg{};
+    int device = 0;
+    char *arena_mem = nullptr;
+    Arena arena;
+    size_t budget_override = 0;
+    BatchConfig bcfg{32, 2, 4, 0};
+    std::vector<Set> sets;
+    std::vector<Slot> slots;
+    std::unordered_map<int64_t, Req> reqs;
+    std::deque<int64_t> pending, admitted;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::thread worker;
+    bool stop = false;
+    int64_t next_id = 1;
+    int64_t completed = 0, launches = 0, rows = 0, launched_reqs = 0, waited = 0;
+    std::vector<double> latencies;
+    std::string worker_error;
+
+    void admit_locked()
+    {
+        const size_t cap = budget_override ? std::min(budget_override, arena.size) : arena.size;
+        while (!pending.empty()) {
+            Req &r = reqs[pending.front()];
+            size_t off = 0;
+            if (!arena.alloc(r.out_bytes, cap, &off)) {
+                if (!r.waited) {
+                    r.waited = true;
+                    ++waited;
+                }
+                return; // FIFO: later requests wait behind the head
+            }
+            r.out_off = off;
+            r.st = St::Admitted;
+            admitted.push_back(pending.front());
+            pending.pop_front();
+        }
+    }
+
+    bool launch_locked(Slot &sl)
+    {
+        std::vector<ibp_fetch_task> tasks;
+        int64_t nrows = 0;
+        sl.reqs.clear();
+        while (!admitted.empty() && (int)sl.reqs.size() < cfg.max_batch_requests) {
+            Req &r = reqs[admitted.front()];
+            if (!sl.reqs.empty() && nrows + r.n > cfg.max_batch_rows) break;
+            const Set &s = sets[r.set];
+            memcpy(sl.host_index + nrows, r.index.data(), sizeof(int64_t) * (size_t)r.n);
+            ibp_fetch_task t;
+            t.comp = s.comp;
+            t.mask = s.mask;
+            t.bitval = s.bitval;
+            t.bitmask = s.bitmask;
+            t.index = sl.dev_index + nrows;
+            t.out = arena_mem + r.out_off;
+            t.num_vecs = s.num_vecs;
+            t.vec_size = s.vec_size;
+            t.n = r.n;
+            t.elem_bytes = s.elem_bytes;
+            t.index_bytes = 8;
+            t.comp_len = s.comp_len;
+            t.comp_loc = s.comp_loc;
+            tasks.push_back(t);
+            nrows += r.n;
+            r.st = St::Running;
+            sl.reqs.push_back(admitted.front());
+            admitted.pop_front();
+        }
+        if (sl.reqs.empty()) return false;
+        const int64_t items = fill_tasks(tasks.data(), (int)tasks.size(), bcfg, sl.host_tasks);
+        hipError_t e = hipMemcpyAsync(sl.dev_index, sl.host_index, sizeof(int64_t) * (size_t)nrows,
+                                      hipMemcpyHostToDevice, sl.stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(sl.dev_tasks, sl.host_tasks, sizeof(TaskDev) * tasks.size(), hipMemcpyHostToDevice,
+                               sl.stream);
+        int st = e == hipSuccess ? launch_batch(sl.dev_tasks, (int)tasks.size(), items, sl.head, bcfg, sl.stream)
+                                 : IBP_ERR_HIP;
+        if (st == IBP_OK && hipEventRecord(sl.done, sl.stream) != hipSuccess) st = IBP_ERR_HIP;
+        if (st != IBP_OK) worker_error = "launch failed";
+        sl.busy = true;
+        ++launches;
+        launched_reqs += (int64_t)sl.reqs.size();
+        rows += nrows;
+        return true;
+    }
+
+    void poll_locked()
+    {
+        for (Slot &sl : slots) {
+            if (!sl.busy) continue;
+            const hipError_t q = hipEventQuery(sl.done);
+            if (q == hipErrorNotReady) continue;
+            const double t = now_us();
+            for (int64_t id : sl.reqs) {
+                Req &r = reqs[id];
+                r.st = St::Done;
+                r.t_done = t;
+                latencies.push_back(r.t_done - r.t_arrival);
+                ++completed;
+            }
+            sl.reqs.clear();
+            sl.busy = false;
+            cv_done.notify_all();
+        }
+    }
+
+    void run()
+    {
+        (void)hipSetDevice(device);
+        std::unique_lock<std::mutex> lk(mu);
+        while (!stop) {
+            poll_locked();
+            admit_locked();
+            for (Slot &sl : slots)
+                if (!sl.busy && !admitted.empty()) launch_locked(sl);
+            bool busy = false;
+            for (Slot &sl : slots) busy = busy || sl.busy;
+            if (!busy && admitted.empty()) {
+                cv_work.wait(lk);
+            } else {
+                lk.unlock();
+                std::this_thread::yield();
+                lk.lock();
+            }
+        }
+    }
+};
+
+extern "C" int ibp_server_create(const ibp_server_config *cfg, ibp_server **out)
+{
+    IBP_CHECK(cfg && out, "null argument");
+    IBP_CHECK(cfg->hbm_budget_bytes >= 4096, "hbm_budget_bytes too small");
+    IBP_CHECK(cfg->num_streams >= 1 && cfg->num_streams <= 16, "num_streams must be in [1, 16]");
+    IBP_CHECK(cfg->max_batch_requests >= 1 && cfg->max_batch_rows >= 1, "batch limits must be positive");
+    ibp_server *srv = new ibp_server();
+    srv->cfg = *cfg;
+    (void)hipGetDevice(&srv->device);
+    srv->bcfg.blocks = device_info().num_cus * 4;
+    auto fail = [&](const char *what, hipError_t e) {
+        ibp_server_destroy(srv);
+        return set_error(IBP_ERR_NOMEM, "server %s failed: %s", what, hipGetErrorString(e));
+    };
+    hipError_t e = hipMalloc((void **)&srv->arena_mem, cfg->hbm_budget_bytes);
+    if (e != hipSuccess) return fail("arena allocation", e);
+    srv->arena.init(cfg->hbm_budget_bytes);
+    srv->slots.resize((size_t)cfg->num_streams);
+    for (Slot &sl : srv->slots) {
+        // Streams keep the default priority.
+        if ((e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking)) != hipSuccess) return fail("stream", e);
+        if ((e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming)) != hipSuccess) return fail("event", e);
+        if ((e = hipMalloc((void **)&sl.dev_tasks, sizeof(TaskDev) * (size_t)cfg->max_batch_requests)) != hipSuccess)
+            return fail("task table", e);
+        if ((e = hipHostMalloc((void **)&sl.host_tasks, sizeof(TaskDev) * (size_t)cfg->max_batch_requests, 0)) !=
+            hipSuccess)
+            return fail("task staging", e);
+        if ((e = hipMalloc((void **)&sl.dev_index, sizeof(int64_t) * (size_t)cfg->max_batch_rows)) != hipSuccess)
+            return fail("index buffer", e);
+        if ((e = hipHostMalloc((void **)&sl.host_index, sizeof(int64_t) * (size_t)cfg->max_batch_rows, 0)) !=
+            hipSuccess)
+            return fail("index staging", e);
+        if ((e = hipMalloc((void **)&sl.head, sizeof(unsigned long long))) != hipSuccess) return fail("queue head", e);
+    }
+    srv->worker = std::thread([srv] { srv->run(); });
+    *out = srv;
+    return IBP_OK;
+}
+
+extern "C" int ibp_server_destroy(ibp_server *srv)
+{
+    if (!srv) return IBP_OK;
+    {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        srv->stop = true;
+    }
+    srv->cv_work.notify_all();
+    if (srv->worker.joinable()) srv->worker.join();
+    for (Slot &sl : srv->slots) {
+        if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+        if (sl.stream) (void)hipStreamDestroy(sl.stream);
+        if (sl.done) (void)hipEventDestroy(sl.done);
+        if (sl.dev_tasks) (void)hipFree(sl.dev_tasks);
+        if (sl.host_tasks) (void)hipHostFree(sl.host_tasks);
+        if (sl.dev_index) (void)hipFree(sl.dev_index);
+        if (sl.host_index) (void)hipHostFree(sl.host_index);
+        if (sl.head) (void)hipFree(sl.head);
+    }
+    if (srv->arena_mem) (void)hipFree(srv->arena_mem);
+    delete srv;
+    return IBP_OK;
+}
+
+extern "C" int ibp_server_register(ibp_server *srv, int elem_bytes, const void *comp, int64_t num_vecs,
+                                   int64_t vec_size, const void *mask, const void *bitval, const int32_t *bitmask,
+                                   int comp_len, int comp_loc, int *set_id)
+{
+    IBP_CHECK(srv && set_id, "null argument");
+    IBP_CHECK(elem_bytes == 4 || elem_bytes == 8, "element size must be 4 or 8");
+    IBP_CHECK(comp && mask && bitval && bitmask && vec_size > 0 && num_vecs >= 0, "invalid tensor set");
+    std::lock_guard<std::mutex> lk(srv->mu);
+    srv->sets.push_back(Set{elem_bytes, comp, num_vecs, vec_size, mask, bitval, bitmask, comp_len, comp_loc});
+    *set_id = (int)srv->sets.size() - 1;
+    return IBP_OK;
+}
+
+extern "C" int ibp_server_submit(ibp_server *srv, int set_id, const int64_t *index, int64_t n, int64_t *req_id)
+{
+    IBP_CHECK(srv && req_id, "null argument");
+    IBP_CHECK(n >= 1 && index != nullptr, "a request needs at least one index");
+    std::unique_lock<std::mutex> lk(srv->mu);
+    IBP_CHECK(set_id >= 0 && set_id < (int)srv->sets.size(), "unknown set %d", set_id);
+    IBP_CHECK(n <= srv->cfg.max_batch_rows, "request of %lld rows exceeds max_batch_rows (%lld)", (long long)n,
+              (long long)srv->cfg.max_batch_rows);
+    const Set &s = srv->sets[set_id];
+    const size_t bytes = (size_t)n * (size_t)s.vec_size * (size_t)s.elem_bytes;
+    if (bytes > srv->arena.size)
+        return set_error(IBP_ERR_NOMEM, "request output (%zu bytes) can never fit the arena (%zu bytes)", bytes,
+                         srv->arena.size);
+    const int64_t id = srv->next_id++;
+    Req &r = srv->reqs[id];
+    r.set = set_id;
+    r.index.assign(index, index + n);
+    r.n = n;
+    r.out_bytes = bytes;
+    r.t_arrival = now_us();
+    srv->pending.push_back(id);
+    lk.unlock();
+    srv->cv_work.notify_all();
+    *req_id = id;
+    return IBP_OK;
+}
+
+extern "C" int ibp_server_wait(ibp_server *srv, int64_t req_id, int timeout_ms, void **out, double *latency_us)
+{
+    IBP_CHECK(srv, "null server");
+    std::unique_lock<std::mutex> lk(srv->mu);
+    auto it = srv->reqs.find(req_id);
+    IBP_CHECK(it != srv->reqs.end(), "unknown request %lld", (long long)req_id);
+    auto ready = [&] { return srv->reqs[req_id].st == St::Done || !srv->worker_error.empty(); };
+    if (timeout_ms < 0) {
+        srv->cv_done.wait(lk, ready);
+    } else if (!srv->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready)) {
+        return set_error(IBP_ERR_BUSY, "request %lld not finished", (long long)req_id);
+    }
+    if (!srv->worker_error.empty()) return set_error(IBP_ERR_HIP, "server worker: %s", srv->worker_error.c_str());
+    const Req &r = srv->reqs[req_id];
+    if (out) *out = srv->arena_mem + r.out_off;
+    if (latency_us) *latency_us = r.t_done - r.t_arrival;
+    return IBP_OK;
+}
+
+extern "C" int ibp_server_release(ibp_server *srv, int64_t req_id)
+{
+    IBP_CHECK(srv, "null server");
+    {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        auto it = srv->reqs.find(req_id);
+        IBP_CHECK(it != srv->reqs.end(), "unknown request %lld", (long long)req_id);
+        IBP_CHECK(it->second.st == St::Done, "request %lld is still in flight", (long long)req_id);
+        srv->arena.release(it->second.out_off, it->second.out_bytes);
+        srv->reqs.erase(it);
+    }
+    srv->cv_work.notify_all();
+    return IBP_OK;
+}
+
+extern "C" int ibp_server_stats_get(ibp_server *srv, ibp_server_stats *st)
+{
+    IBP_CHECK(srv && st, "null argument");
+    std::lock_guard<std::mutex> lk(srv->mu);
+    st->completed = srv->completed;
+    st->launches = srv->launches;
+    st->rows = srv->rows;
+    std::vector<double> l = srv->latencies;
+    std::sort(l.begin(), l.end());
+    auto pct = [&](double p) { return l.empty() ? 0.0 : l[(size_t)std::min<double>(l.size() - 1, p * (l.size() - 1) + 0.5)]; };
+    st->p50_us = pct(0.50);
+    st->p99_us = pct(0.99);
+    st->mean_batch_requests = srv->launches ? (double)srv->launched_reqs / (double)srv->launches : 0.0;
+    st->peak_arena_bytes = srv->arena.peak;
+    st->waited_admissions = srv->waited;
+    return IBP_OK;
+}
+
+extern "C" int ibp_server_set_budget_override(ibp_server *srv, size_t bytes)
+{
+    IBP_CHECK(srv, "null server");
+    {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        srv->budget_override = bytes;
+    }
+    srv->cv_work.notify_all();
+    return IBP_OK;
+}

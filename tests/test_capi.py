@@ -1,0 +1,118 @@
+"""CPU tests of the C ABI boundary: libibp.so loads, exports every symbol that
+include/ibp.h declares, its host-compiled bit arithmetic matches a brute-force
+model, and argument errors come back as status codes (no GPU needed)."""
+import ctypes
+import os
+import re
+
+import pytest
+import torch
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def capi():
+    from invariantbitpacking_amd import _capi
+    _capi.lib()
+    return _capi
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "ibp.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(ibp_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_exports_every_declared_symbol(capi):
+    names = declared_functions()
+    assert len(names) >= 25
+    for name in names:
+        assert hasattr(capi.lib(), name), name
+        assert name in capi.SIGNATURES, "no ctypes signature for " + name
+
+
+def test_version_and_debug(capi):
+    assert b"gfx950" in capi.lib().ibp_version()
+    capi.lib().ibp_set_debug(1)
+    capi.lib().ibp_set_debug(0)
+
+
+def test_selftest_bitops_host(capi):
+    assert capi.lib().ibp_selftest_bitops(50000, 12345) == 0, capi.last_error()
+
+
+def test_invalid_arguments_are_status_codes(capi):
+    L = capi.lib()
+    st = L.ibp_decompress_fetch(3, None, None, 10, 4, None, None, None, None, 0, 10, 0, 0, 0, -1, 0, None)
+    assert st == 1 and "4-byte or 8-byte" in capi.last_error()
+    st = L.ibp_decompress_fetch(8, None, None, 10, 4, None, None, None, None, 0, 10, 0, 0, 0, 9, 0, None)
+    assert st == 1 and "impl" in capi.last_error()
+    st = L.ibp_decompress_fetch(8, None, None, 10, 4, None, None, None, None, 0, 11, 0, 0, 0, -1, 0, None)
+    assert st == 1
+    # n == 0 is a no-op
+    assert L.ibp_decompress_fetch(8, None, None, 10, 4, None, None, None, None, 0, 0, 0, 0, 0, -1, 0, None) == 0
+    buf = ctypes.create_string_buffer(64)
+    st = L.ibp_compress(8, buf, buf, 2, 4, buf, buf, buf, 8, 2, buf, None)
+    assert st == 1 and "alias" in capi.last_error()
+    st = L.ibp_compress(2, buf, buf, 2, 4, buf, buf, None, 0, 2, buf, None)
+    assert st == 1
+    st = L.ibp_preprocess(3, buf, 2, 4, 0.9, buf, buf, None, None, 0, None)
+    assert st == 1 and "1, 2, 4, or 8-byte" in capi.last_error()
+    st = L.ibp_preprocess_kmeans(8, buf, 2, 4, 0, 0.9, 1, buf, buf, buf, None)
+    assert st == 1 and "num_clusters" in capi.last_error()
+    st = L.ibp_get_compress_size(8, buf, 2, 4, buf, buf, None, 0, 3, buf, None, None)
+    assert st == 1
+    plan = ctypes.c_void_p()
+    assert L.ibp_plan_create(None, 0, ctypes.byref(plan)) == 1
+    cfg = capi.ServerConfig(16, 1, 1, 1)
+    srv = ctypes.c_void_p()
+    assert L.ibp_server_create(ctypes.byref(cfg), ctypes.byref(srv)) == 1
+
+
+def test_preprocess_workspace_size(capi):
+    L = capi.lib()
+    sweep = L.ibp_preprocess_workspace_size(8, 64, -1.0)
+    single = L.ibp_preprocess_workspace_size(8, 64, 0.9)
+    assert sweep > single >= 64 * 64 * 4
+
+
+def test_python_api_rejects_unreachable_memory():
+    """Plain (pageable) CPU tensors are not reachable by the GPU: the same
+    RuntimeError as the reference's TORCH_CHECK, raised before any HIP call."""
+    import invariantbitpacking_amd as ibp
+    x = torch.zeros(4, 8, dtype=torch.int64)
+    m = torch.zeros(8, dtype=torch.int64)
+    with pytest.raises(RuntimeError, match="accessible by CUDA device"):
+        ibp.decompress_fetch(x, m, m, torch.zeros(1, dtype=torch.int32), torch.device("cuda"))
+    with pytest.raises(RuntimeError, match="accessible by CUDA device"):
+        ibp.preprocess(x)
+    with pytest.raises(RuntimeError, match="accessible by CUDA device"):
+        ibp.compress_inplace(x, m, m)
+    with pytest.raises(RuntimeError, match="accessible by CUDA device"):
+        ibp.get_compress_size(x, m, m)
+    with pytest.raises(NotImplementedError):
+        ibp.compress(x, m, m)
+
+
+def test_reference_api_surface():
+    """Names and positional argument order of ibp/__init__.py:8-41."""
+    import inspect
+
+    import invariantbitpacking_amd as ibp
+    want = {
+        "print_debug": ["flag"],
+        "preprocess": ["dataset", "threshold"],
+        "preprocess_kmeans": ["dataset", "centroids", "threshold"],
+        "get_compress_size": ["dataset", "mask", "bitval", "index_arr", "compress_total"],
+        "get_single_compress_len": ["dataset", "mask", "bitval", "index_arr"],
+        "compress_inplace": ["dataset", "mask", "bitval", "index_arr"],
+        "compress": ["dataset", "mask", "bitval", "index_arr"],
+        "decompress_fetch": ["comp_dataset", "mask", "bitval", "bitmask", "device", "comp_len", "index_arr",
+                             "nblks", "nthds", "imp", "output_tensor"],
+    }
+    for name, params in want.items():
+        sig = inspect.signature(getattr(ibp, name))
+        positional = [p.name for p in sig.parameters.values()
+                      if p.kind in (p.POSITIONAL_ONLY, p.POSITIONAL_OR_KEYWORD)]
+        assert positional == params, name

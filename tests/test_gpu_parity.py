@@ -1,0 +1,317 @@
+"""GPU parity: the HIP kernels (through the C ABI) against the CPU oracle,
+bit-exact, on seeded inputs; plus size-independent properties at full size."""
+import json
+import os
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "known_answers.json")
+
+
+@pytest.fixture(scope="module")
+def ibp():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import invariantbitpacking_amd as m
+    m.index_errors(reset=True)
+    return m
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _u(a):
+    return a.view(np.dtype("u%d" % a.itemsize))
+
+
+def float_rows(n, v, w, seed, zero_frac=0.3, sd=0.1):
+    """GNN-like rows: floats with exact zeros, viewed as w-byte integers."""
+    rng = np.random.default_rng(seed)
+    f = rng.normal(0, sd, size=(n, v * w // 4 if w >= 4 else v)).astype(np.float32)
+    f[rng.random(f.shape) < zero_frac] = 0.0
+    raw = f.view(np.uint8).reshape(n, -1)[:, : v * w]
+    return np.ascontiguousarray(raw).view(np.dtype("i%d" % w))
+
+
+def place(arr, where):
+    t = torch.from_numpy(np.ascontiguousarray(arr))
+    return t.cuda() if where == "device" else t.pin_memory()
+
+
+# ------------------------------------------------------------------ preprocess
+
+@pytest.mark.parametrize("w", [1, 2, 4, 8])
+@pytest.mark.parametrize("threshold", [None, 0.9])
+@pytest.mark.parametrize("shape", [(1000, 64), (257, 9), (300, 130), (64, 1)])
+def test_preprocess_matches_oracle(ibp, oracle, w, threshold, shape):
+    n, v = shape
+    data = float_rows(n, v, w, seed=n + v + w)
+    for where in ("device", "pinned"):
+        print("preprocess", w, threshold, shape, where, flush=True)
+        m, b = ibp.preprocess(place(data, where), threshold)
+        om, ob, _, _ = oracle.preprocess(data, threshold)
+        assert np.array_equal(_u(_np(m)), om) and np.array_equal(_u(_np(b)), ob)
+        assert m.dtype == {1: torch.int8, 2: torch.int16, 4: torch.int32, 8: torch.int64}[w]
+
+
+def test_preprocess_golden(ibp, oracle):
+    with open(GOLDEN) as f:
+        cases = json.load(f)["preprocess"]
+    for c in cases:
+        rows = np.array([[int(x, 16) for x in r] for r in c["rows"]], dtype=np.uint8).view(np.int8)
+        m, b = ibp.preprocess(place(rows, "device"), c["threshold"])
+        assert [hex(x) for x in _u(_np(m))] == [hex(int(x, 16)) for x in c["mask"]], c["name"]
+        assert [hex(x) for x in _u(_np(b))] == [hex(int(x, 16)) for x in c["bitval"]], c["name"]
+
+
+# ------------------------------------------------------------------ size + pack
+
+SHAPES = [(200, 1), (200, 3), (500, 50), (700, 64), (300, 65), (200, 130), (64, 1024), (40, 2752)]
+
+
+@pytest.mark.parametrize("w", [4, 8])
+@pytest.mark.parametrize("shape", SHAPES)
+def test_size_and_compress_match_oracle(ibp, oracle, w, shape):
+    n, v = shape
+    data = float_rows(n, v, w, seed=7 * n + v + w)
+    om, ob, _, _ = oracle.preprocess(data, None)
+    mask = torch.from_numpy(om.view(np.dtype("i%d" % w))).cuda()
+    bitval = torch.from_numpy(ob.view(np.dtype("i%d" % w))).cuda()
+    osz, _ = oracle.get_compress_size(data, om, ob)
+    ocomp, obm = oracle.compress(data, om, ob)
+    for where in ("device", "pinned"):
+        print("compress", w, shape, where, flush=True)
+        t = place(data, where)
+        sz = ibp.get_compress_size(t, mask, bitval)
+        torch.cuda.synchronize()
+        assert np.array_equal(_np(sz), osz)
+        bm = ibp.compress_inplace(t, mask, bitval)
+        torch.cuda.synchronize()
+        assert np.array_equal(_np(bm), obm), "bitmask"
+        assert np.array_equal(_u(_np(t)), _u(ocomp)), "slots"
+
+
+def test_golden_slots_on_gpu(ibp):
+    with open(GOLDEN) as f:
+        cases = json.load(f)["slots"]
+    for c in cases:
+        w = c["w"]
+        dt = np.dtype("u%d" % w)
+        row = np.array([int(x, 16) for x in c["row"]], dtype=dt)[None, :].view(np.dtype("i%d" % w))
+        mask = torch.from_numpy(np.array([int(x, 16) for x in c["mask"]], dtype=dt).view(row.dtype)).cuda()
+        bitval = torch.from_numpy(np.array([int(x, 16) for x in c["bitval"]], dtype=dt).view(row.dtype)).cuda()
+        t = place(row, "pinned")
+        sizes = ibp.get_compress_size(t, mask, bitval)
+        torch.cuda.synchronize()  # asynchronous, like the reference
+        assert int(sizes[0]) == c["size"], c["name"]
+        bm = ibp.compress_inplace(t, mask, bitval)
+        torch.cuda.synchronize()
+        assert [hex(x) for x in _u(_np(t))[0]] == [hex(int(x, 16)) for x in c["slot"]], c["name"]
+        out = ibp.decompress_fetch(t, mask, bitval, bm, torch.device("cuda"))
+        assert np.array_equal(_np(out), row), c["name"]
+
+
+def test_size_with_index_and_counter(ibp, oracle):
+    data = float_rows(400, 64, 8, seed=3)
+    om, ob, _, _ = oracle.preprocess(data, 0.9)
+    mask, bitval = (torch.from_numpy(a.view(np.int64)).cuda() for a in (om, ob))
+    idx = np.array([5, 0, 399, 5, 12, 77], dtype=np.int64)
+    osz, _ = oracle.get_compress_size(data, om, ob, idx)
+    for where in ("device", "pinned"):
+        t = place(data, where)
+        for itype in (torch.int32, torch.int64):
+            ctr = torch.tensor([100], dtype=torch.int64, device="cuda")
+            sz = ibp.get_compress_size(t, mask, bitval, torch.from_numpy(idx).to(itype).cuda(), ctr)
+            torch.cuda.synchronize()
+            assert np.array_equal(_np(sz), osz)
+            assert int(ctr[0]) == 100 + int(osz.sum())
+        pinned_ctr = torch.tensor([7], dtype=torch.int64).pin_memory()
+        ibp.get_compress_size(t, mask, bitval, None, pinned_ctr)
+        full, _ = oracle.get_compress_size(data, om, ob)
+        assert int(pinned_ctr[0]) == 7 + int(full.sum())
+    L = ibp.get_single_compress_len(place(data, "device"), mask, bitval)
+    assert L == int(int(full.sum()) / data.nbytes * 64)
+
+
+def test_compress_with_index(ibp, oracle):
+    data = float_rows(128, 64, 8, seed=9)
+    om, ob, _, _ = oracle.preprocess(data, None)
+    mask, bitval = (torch.from_numpy(a.view(np.int64)).cuda() for a in (om, ob))
+    idx = np.array([3, 3, 100, 1, 0, 64, 127], dtype=np.int64)
+    ocomp, obm = oracle.compress(data, om, ob, idx)
+    t = place(data, "pinned")
+    bm = ibp.compress_inplace(t, mask, bitval, torch.from_numpy(idx).cuda())
+    torch.cuda.synchronize()
+    assert np.array_equal(_np(bm), obm)
+    assert np.array_equal(_u(_np(t)[: len(idx)]), _u(ocomp))
+
+
+# ------------------------------------------------------------------ fetch
+
+def _compressed(ibp, oracle, n, v, w, seed, where, threshold=None):
+    data = float_rows(n, v, w, seed=seed)
+    om, ob, _, _ = oracle.preprocess(data, threshold)
+    mask = torch.from_numpy(om.view(np.dtype("i%d" % w))).cuda()
+    bitval = torch.from_numpy(ob.view(np.dtype("i%d" % w))).cuda()
+    ocomp, obm = oracle.compress(data, om, ob)
+    comp = place(ocomp.view(np.dtype("i%d" % w)), where)
+    bitmask = torch.from_numpy(obm).cuda()
+    return data, comp, mask, bitval, bitmask, (om, ob, ocomp, obm)
+
+
+@pytest.mark.parametrize("w", [4, 8])
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("where", ["device", "pinned"])
+def test_fetch_matches_oracle(ibp, oracle, w, shape, where):
+    n, v = shape
+    data, comp, mask, bitval, bitmask, (om, ob, ocomp, obm) = _compressed(ibp, oracle, n, v, w, 11 * n + v, where)
+    rng = np.random.default_rng(v)
+    idx = rng.integers(0, n, size=3 * n + 5)  # duplicates, unsorted
+    want = oracle.decompress_fetch(ocomp, om, ob, obm, idx)
+    assert np.array_equal(_u(want), _u(data[idx]))
+    L = ibp.get_single_compress_len(comp, mask, bitval)
+    for itype in (torch.int64, torch.int32):
+        for comp_len in (None, L, 1, v):
+            print("fetch", w, shape, where, itype, comp_len, flush=True)
+            out = ibp.decompress_fetch(comp, mask, bitval, bitmask, torch.device("cuda"), comp_len,
+                                       torch.from_numpy(idx).to(itype).cuda())
+            torch.cuda.synchronize()
+            assert np.array_equal(_u(_np(out)), _u(want))
+    out = ibp.decompress_fetch(comp, mask, bitval, bitmask, torch.device("cuda"))
+    assert np.array_equal(_np(out), data)
+
+
+@pytest.mark.parametrize("imp", [-1, 0, 1, 5])
+def test_fetch_impls_identical(ibp, oracle, imp):
+    data, comp, mask, bitval, bitmask, _ = _compressed(ibp, oracle, 333, 64, 8, 5, "pinned")
+    idx = torch.randint(0, 333, (1000,), generator=torch.Generator().manual_seed(imp + 2)).cuda()
+    out = ibp.decompress_fetch(comp, mask, bitval, bitmask, torch.device("cuda"), None, idx, 320, 512, imp)
+    assert np.array_equal(_np(out), data[_np(idx)])
+
+
+def test_fetch_raw_gather_impl2(ibp, oracle):
+    data, comp, mask, bitval, bitmask, (om, ob, ocomp, obm) = _compressed(ibp, oracle, 200, 64, 8, 6, "pinned")
+    idx = torch.arange(199, -1, -1).cuda()
+    out = ibp.decompress_fetch(comp, mask, bitval, bitmask, torch.device("cuda"), None, idx, None, None, 2)
+    assert np.array_equal(_u(_np(out)), _u(ocomp[_np(idx)]))
+
+
+def test_fetch_output_tensor_and_host_output(ibp, oracle):
+    data, comp, mask, bitval, bitmask, _ = _compressed(ibp, oracle, 500, 50, 8, 8, "device")
+    idx = torch.randint(0, 500, (777,), generator=torch.Generator().manual_seed(3))
+    out = torch.full((777, 50), -1, dtype=torch.int64, device="cuda")
+    r = ibp.decompress_fetch(comp, mask, bitval, bitmask, torch.device("cuda"), None, idx.cuda(), None, None, None,
+                             out)
+    assert r.data_ptr() == out.data_ptr()
+    assert np.array_equal(_np(out), data[idx.numpy()])
+    host = ibp.decompress_fetch(comp, mask, bitval, bitmask, torch.device("cpu"), None, idx.cuda())
+    torch.cuda.synchronize()
+    assert host.device.type == "cpu" and host.is_pinned()
+    assert np.array_equal(host.numpy(), data[idx.numpy()])
+    with pytest.raises(RuntimeError, match="shape"):
+        ibp.decompress_fetch(comp, mask, bitval, bitmask, torch.device("cuda"), None, idx.cuda(), None, None, None,
+                             torch.empty((776, 50), dtype=torch.int64, device="cuda"))
+    with pytest.raises(RuntimeError, match="Index array"):
+        ibp.decompress_fetch(comp, mask, bitval, bitmask, torch.device("cuda"), None, idx)
+
+
+def test_fetch_edge_cases(ibp, oracle):
+    data, comp, mask, bitval, bitmask, _ = _compressed(ibp, oracle, 100, 64, 8, 10, "pinned")
+    empty = ibp.decompress_fetch(comp, mask, bitval, bitmask, torch.device("cuda"), None,
+                                 torch.empty(0, dtype=torch.int64, device="cuda"))
+    assert tuple(empty.shape) == (0, 64)
+    ibp.index_errors(reset=True)
+    idx = torch.tensor([0, 100, -1, 99, 1 << 40], dtype=torch.int64, device="cuda")
+    out = _np(ibp.decompress_fetch(comp, mask, bitval, bitmask, torch.device("cuda"), None, idx))
+    assert np.array_equal(out[0], data[0]) and np.array_equal(out[3], data[99])
+    assert not out[1].any() and not out[2].any() and not out[4].any()
+    assert ibp.index_errors(reset=True) == 3
+    # single row, 1-D dataset
+    one = torch.from_numpy(data[7].copy()).pin_memory()
+    bm1 = ibp.compress_inplace(one, mask, bitval)
+    out1 = ibp.decompress_fetch(one, mask, bitval, bm1, torch.device("cuda"))
+    assert tuple(out1.shape) == (1, 64) and np.array_equal(_np(out1)[0], data[7])
+
+
+def test_fetch_full_width_fields(ibp, oracle):
+    """Raw 64-bit elements at 64-aligned stream positions and flagged elements
+    with mask 0 (the full-width shift cases) through compress + fetch."""
+    rng = np.random.default_rng(0)
+    v = 64
+    mask = np.zeros(v, dtype=np.uint64)
+    mask[1::2] = np.uint64(0xFFFFFFFFFFFFFFF0)
+    data = rng.integers(0, 1 << 62, size=(256, v), dtype=np.uint64)
+    data[:, 1::2] &= np.uint64(0xF)  # flagged under mask, 4 free bits each
+    data[::3, 1::4] |= np.uint64(1 << 63)  # some unflagged -> raw 64-bit fields
+    bitval = np.zeros(v, dtype=np.uint64)
+    ocomp, obm = oracle.compress(data, mask, bitval)
+    assert obm.any()
+    m = torch.from_numpy(mask.view(np.int64)).cuda()
+    b = torch.from_numpy(bitval.view(np.int64)).cuda()
+    t = place(data.view(np.int64), "pinned")
+    bm = ibp.compress_inplace(t, m, b)
+    torch.cuda.synchronize()
+    assert np.array_equal(_u(_np(t)), ocomp) and np.array_equal(_np(bm), obm)
+    out = ibp.decompress_fetch(t, m, b, bm, torch.device("cuda"))
+    assert np.array_equal(_u(_np(out)), data)
+
+
+# ------------------------------------------------------------------ kmeans
+
+@pytest.mark.parametrize("w,n,v,k", [(4, 60, 40, 4), (8, 60, 70, 3), (8, 300, 64, 16), (4, 10, 5, 16)])
+def test_kmeans_matches_oracle(ibp, oracle, w, n, v, k):
+    rng = np.random.default_rng(n + v + k)
+    dt = np.dtype("u%d" % w)
+    base = rng.integers(0, 1 << 62, size=(4, v), dtype=np.uint64).astype(dt)
+    data = base[rng.integers(0, 4, size=n)] ^ rng.integers(0, 16, size=(n, v), dtype=np.uint64).astype(dt)
+    data = data.view(np.dtype("i%d" % w))
+    om, ob, ocl = oracle.kmeans(data, k, seed=3)
+    for where in ("device", "pinned"):
+        print("kmeans", w, n, v, k, where, flush=True)
+        m, b, cl = ibp.preprocess_kmeans(place(data, where), k, 0.85, seed=3)
+        assert cl.is_pinned() and cl.dtype == torch.int32
+        assert np.array_equal(cl.numpy(), ocl)
+        assert np.array_equal(_u(_np(m)), om) and np.array_equal(_u(_np(b)), ob)
+
+
+def test_kmeans_mask_row0_used_by_compress(ibp, oracle):
+    """A [K, V] mask is used through its first row, as in the reference."""
+    data = float_rows(64, 16, 8, seed=1)
+    m, b, _ = ibp.preprocess_kmeans(place(data, "device"), 4, seed=1)
+    om, ob = _u(_np(m))[0], _u(_np(b))[0]
+    ocomp, obm = oracle.compress(data, om, ob)
+    t = place(data, "pinned")
+    bm = ibp.compress_inplace(t, m, b)
+    torch.cuda.synchronize()
+    assert np.array_equal(_np(bm), obm) and np.array_equal(_u(_np(t)), _u(ocomp))
+
+
+# ------------------------------------------------------------------ full size
+
+def test_gnn_features_full_size_round_trip(ibp, oracle):
+    """65536 x 64 int64 (fp32 N(0,0.1), 30% zeros) at threshold 0.9, pinned,
+    4096 random indices: the fetched rows equal the original rows."""
+    from invariantbitpacking_amd import workloads
+    data, idx = workloads.gnn_features()
+    x = data.numpy()
+    pinned = data.clone().pin_memory()
+    mask, bitval = ibp.preprocess(pinned, 0.9)
+    om, ob, _, _ = oracle.preprocess(x, 0.9)
+    assert np.array_equal(_u(_np(mask)), om) and np.array_equal(_u(_np(bitval)), ob)
+    sizes = ibp.get_compress_size(pinned, mask, bitval)
+    bm = ibp.compress_inplace(pinned, mask, bitval)
+    torch.cuda.synchronize()
+    s = sizes.numpy()
+    assert np.all(s <= 512) and np.all(s % 8 == 0)
+    assert np.array_equal(np.unpackbits(_np(bm).view(np.uint8), bitorder="little")[: len(s)].astype(bool), s < 512)
+    L = ibp.get_single_compress_len(data.clone().pin_memory(), mask, bitval)
+    out = ibp.decompress_fetch(pinned, mask, bitval, bm, torch.device("cuda"), L, idx.cuda())
+    assert torch.equal(out.cpu(), data[idx])
+    dev = pinned.cuda()
+    out2 = ibp.decompress_fetch(dev, mask, bitval, bm, torch.device("cuda"), L, idx.cuda())
+    assert torch.equal(out2.cpu(), data[idx])
