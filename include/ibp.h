@@ -117,6 +117,11 @@ int ibp_decompress_fetch(int elem_bytes, void *out, const void *comp, int64_t nu
                          int comp_len, int nblks, int nthreads, int impl, int comp_loc,
                          void *stream);
 
+/* Launch shape of ibp_decompress_fetch for rows of <= 64 elements: rows kept
+ * in flight per wave (2, 4, 8 or 16) and resident blocks per CU (1..32).
+ * Process-wide; values are unaffected. */
+int ibp_set_fetch_tuning(int rows_in_flight, int blocks_per_cu);
+
 /* ---- batched fetch (no reference counterpart: BASELINE's batched form) ------ */
 
 /* One table fetch of a batch: fields as in ibp_decompress_fetch. */

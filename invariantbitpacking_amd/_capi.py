@@ -47,6 +47,7 @@ SIGNATURES = {
     "ibp_compress": (_I32, [_I32, _P, _P, _I64, _I64, _P, _P, _P, _I32, _I64, _P, _P]),
     "ibp_decompress_fetch": (_I32, [_I32, _P, _P, _I64, _I64, _P, _P, _P, _P, _I32, _I64, _I32, _I32, _I32,
                                     _I32, _I32, _P]),
+    "ibp_set_fetch_tuning": (_I32, [_I32, _I32]),
     "ibp_plan_create": (_I32, [ctypes.POINTER(FetchTask), _I32, ctypes.POINTER(_P)]),
     "ibp_plan_destroy": (_I32, [_P]),
     "ibp_plan_launch": (_I32, [_P, _P]),

@@ -23,6 +23,7 @@ struct BatchConfig {
     int rows_per_item_large; // rows of a work item, wider tables
     int rows_in_flight;      // 2, 4 or 8 (small rows)
     int blocks;              // persistent grid (256 threads per block)
+    int claim;               // work items claimed per queue atomic
 };
 
 // Fills `out[0..n)` from the public task descriptions; returns the number of

@@ -26,29 +26,25 @@ __global__ void batch_reset_kernel(unsigned long long *head)
     if (threadIdx.x == 0 && blockIdx.x == 0) *head = 0;
 }
 
+// Rows [b, e) of task t.  Column tables of small-row tasks are cached across
+// consecutive items of the same task (cached_task).
 template <int ROWS>
-__device__ __forceinline__ void run_item(const TaskDev &t, int64_t b, int64_t e)
+__device__ __forceinline__ void run_item(const TaskDev &t, int task, int64_t b, int64_t e, int &cached_task,
+                                         SmallCols<uint64_t> &c64, SmallCols<uint32_t> &c32)
 {
     const FetchParams &p = t.p;
     const bool i32 = p.index != nullptr && t.index_bytes == 4;
     if (t.small) {
         if (t.elem_bytes == 8) {
-            const SmallCols<uint64_t> col = load_small_cols<uint64_t>(p.mask, p.bitval, (int)p.vec_size);
-            for (int64_t base = b; base < e; base += ROWS) {
-                FetchParams q = p;
-                q.n = e; // rows beyond the item end are not this wave's
-                if (i32) fetch_small_group<uint64_t, int32_t, ROWS>(q, col, base);
-                else fetch_small_group<uint64_t, int64_t, ROWS>(q, col, base);
-            }
+            if (cached_task != task) c64 = load_small_cols<uint64_t>(p.mask, p.bitval, (int)p.vec_size);
+            if (i32) fetch_small_range<uint64_t, int32_t, ROWS>(p, c64, b, e, ROWS);
+            else fetch_small_range<uint64_t, int64_t, ROWS>(p, c64, b, e, ROWS);
         } else {
-            const SmallCols<uint32_t> col = load_small_cols<uint32_t>(p.mask, p.bitval, (int)p.vec_size);
-            for (int64_t base = b; base < e; base += ROWS) {
-                FetchParams q = p;
-                q.n = e;
-                if (i32) fetch_small_group<uint32_t, int32_t, ROWS>(q, col, base);
-                else fetch_small_group<uint32_t, int64_t, ROWS>(q, col, base);
-            }
+            if (cached_task != task) c32 = load_small_cols<uint32_t>(p.mask, p.bitval, (int)p.vec_size);
+            if (i32) fetch_small_range<uint32_t, int32_t, ROWS>(p, c32, b, e, ROWS);
+            else fetch_small_range<uint32_t, int64_t, ROWS>(p, c32, b, e, ROWS);
         }
+        cached_task = task;
     } else {
         for (int64_t i = b; i < e; ++i) {
             if (t.elem_bytes == 8) {
@@ -62,28 +58,36 @@ __device__ __forceinline__ void run_item(const TaskDev &t, int64_t b, int64_t e)
     }
 }
 
+// Persistent kernel: each wave claims `claim` consecutive work items per
+// atomic on the queue head until the queue is drained.
 template <int ROWS>
 __global__ __launch_bounds__(256) void batch_kernel(const TaskDev *tasks, int ntasks, int64_t total_items,
-                                                    unsigned long long *head)
+                                                    unsigned long long *head, int claim)
 {
     const int lane = lane_id();
+    int cached_task = -1;
+    SmallCols<uint64_t> c64;
+    SmallCols<uint32_t> c32;
     for (;;) {
         unsigned long long claimed = 0;
-        if (lane == 0) claimed = atomicAdd(head, 1ull);
-        const int64_t item = readfirstlane((int64_t)claimed);
-        if (item >= total_items) break; // every wave reaches this exit
-        // Task of the item: last task whose item_begin <= item.
-        int lo = 0, hi = ntasks - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) / 2;
-            if (tasks[mid].item_begin <= item) lo = mid;
-            else hi = mid - 1;
+        if (lane == 0) claimed = atomicAdd(head, (unsigned long long)claim);
+        const int64_t first = readfirstlane((int64_t)claimed);
+        if (first >= total_items) break; // every wave reaches this exit
+        const int64_t last = first + claim < total_items ? first + claim : total_items;
+        for (int64_t item = first; item < last; ++item) {
+            // Task of the item: last task whose item_begin <= item.
+            int lo = 0, hi = ntasks - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) / 2;
+                if (tasks[mid].item_begin <= item) lo = mid;
+                else hi = mid - 1;
+            }
+            const TaskDev t = tasks[lo];
+            const int64_t b = (item - t.item_begin) * t.rows_per_item;
+            int64_t e = b + t.rows_per_item;
+            if (e > t.p.n) e = t.p.n;
+            run_item<ROWS>(t, lo, b, e, cached_task, c64, c32);
         }
-        const TaskDev t = tasks[lo];
-        const int64_t b = (item - t.item_begin) * t.rows_per_item;
-        int64_t e = b + t.rows_per_item;
-        if (e > t.p.n) e = t.p.n;
-        run_item<ROWS>(t, b, e);
     }
 }
 
@@ -122,9 +126,9 @@ int launch_batch(const TaskDev *dev_tasks, int ntasks, int64_t total_items, unsi
     hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, s, head);
     const dim3 grid((unsigned)cfg.blocks), block(256);
     switch (cfg.rows_in_flight) {
-    case 2: hipLaunchKernelGGL((batch_kernel<2>), grid, block, 0, s, dev_tasks, ntasks, total_items, head); break;
-    case 8: hipLaunchKernelGGL((batch_kernel<8>), grid, block, 0, s, dev_tasks, ntasks, total_items, head); break;
-    default: hipLaunchKernelGGL((batch_kernel<4>), grid, block, 0, s, dev_tasks, ntasks, total_items, head); break;
+    case 2: hipLaunchKernelGGL((batch_kernel<2>), grid, block, 0, s, dev_tasks, ntasks, total_items, head, cfg.claim); break;
+    case 8: hipLaunchKernelGGL((batch_kernel<8>), grid, block, 0, s, dev_tasks, ntasks, total_items, head, cfg.claim); break;
+    default: hipLaunchKernelGGL((batch_kernel<4>), grid, block, 0, s, dev_tasks, ntasks, total_items, head, cfg.claim); break;
     }
     IBP_LAUNCH_CHECK();
     return IBP_OK;
@@ -140,7 +144,7 @@ struct ibp_plan {
     TaskDev *dev_tasks = nullptr;
     unsigned long long *head = nullptr;
     int64_t total_items = 0;
-    BatchConfig cfg{32, 2, 4, 0};
+    BatchConfig cfg{128, 2, 8, 0, 4};
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     hipStream_t capture_stream = nullptr;
@@ -228,9 +232,11 @@ extern "C" int ibp_plan_autotune(ibp_plan *pl, void *stream, float *best_ms)
     hipStream_t s = as_stream(stream);
     const DeviceInfo &dev = device_info();
     std::vector<BatchConfig> cands;
-    for (int rpi : {16, 32, 64, 128})
-        for (int rif : {2, 4, 8})
-            for (int mult : {4, 8, 16}) cands.push_back(BatchConfig{rpi, rpi >= 64 ? 4 : 2, rif, dev.num_cus * mult});
+    for (int rpi : {64, 128, 256})
+        for (int rif : {4, 8})
+            for (int mult : {8, 16})
+                for (int claim : {1, 4, 16})
+                    cands.push_back(BatchConfig{rpi, rpi >= 128 ? 4 : 2, rif, dev.num_cus * mult, claim});
     hipEvent_t e0, e1;
     IBP_HIP(hipEventCreate(&e0));
     IBP_HIP(hipEventCreate(&e1));
@@ -300,8 +306,8 @@ extern "C" int ibp_plan_describe(ibp_plan *pl, char *buf, size_t len)
     IBP_CHECK(pl != nullptr && buf != nullptr, "null argument");
     snprintf(buf, len,
              "{\"tasks\": %zu, \"items\": %lld, \"rows_per_item_small\": %d, \"rows_per_item_large\": %d, "
-             "\"rows_in_flight\": %d, \"blocks\": %d, \"graph\": %s}",
+             "\"rows_in_flight\": %d, \"blocks\": %d, \"claim\": %d, \"graph\": %s}",
              pl->tasks.size(), (long long)pl->total_items, pl->cfg.rows_per_item_small, pl->cfg.rows_per_item_large,
-             pl->cfg.rows_in_flight, pl->cfg.blocks, pl->graph_exec ? "true" : "false");
+             pl->cfg.rows_in_flight, pl->cfg.blocks, pl->cfg.claim, pl->graph_exec ? "true" : "false");
     return IBP_OK;
 }

@@ -8,6 +8,8 @@
 //  * no per-call device queries: properties are cached per device;
 //  * out-of-range indices give zero rows and bump a device counter instead of
 //    reading out of bounds.
+#include <atomic>
+
 #include "ibp_fetch_dev.h"
 #include "ibp_internal.h"
 
@@ -19,8 +21,7 @@ __global__ __launch_bounds__(256) void fetch_small_kernel(FetchParams p)
     const SmallCols<T> col = load_small_cols<T>(p.mask, p.bitval, (int)p.vec_size);
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
-    for (int64_t base = wave * ROWS; base < p.n; base += nwaves * ROWS)
-        fetch_small_group<T, IdxT, ROWS>(p, col, base);
+    fetch_small_range<T, IdxT, ROWS>(p, col, wave * ROWS, p.n, nwaves * ROWS);
 }
 
 template <typename T, typename IdxT>
@@ -79,11 +80,10 @@ __global__ __launch_bounds__(256) void fetch_baseline_kernel(FetchParams p, int 
 
 namespace {
 
-int pick_rows(int64_t n)
-{
-    (void)n;
-    return 4;
-}
+// Launch shape of the plain fetch (ibp_set_fetch_tuning): rows in flight
+// per wave for rows of <= 64 elements, and resident blocks per CU.
+std::atomic<int> g_rows_in_flight{8};
+std::atomic<int> g_blocks_per_cu{16};
 
 template <typename T, typename IdxT>
 int launch_typed(const FetchParams &p, int impl, int comp_len, int grid, int block, hipStream_t stream)
@@ -91,10 +91,11 @@ int launch_typed(const FetchParams &p, int impl, int comp_len, int grid, int blo
     if (impl == 2 || impl == 3 || impl == 4) {
         hipLaunchKernelGGL((fetch_baseline_kernel<T, IdxT>), dim3(grid), dim3(block), 0, stream, p, impl, comp_len);
     } else if (p.vec_size <= kWave) {
-        switch (pick_rows(p.n)) {
-        case 8: hipLaunchKernelGGL((fetch_small_kernel<T, IdxT, 8>), dim3(grid), dim3(block), 0, stream, p); break;
+        switch (g_rows_in_flight.load()) {
+        case 16: hipLaunchKernelGGL((fetch_small_kernel<T, IdxT, 16>), dim3(grid), dim3(block), 0, stream, p); break;
+        case 4: hipLaunchKernelGGL((fetch_small_kernel<T, IdxT, 4>), dim3(grid), dim3(block), 0, stream, p); break;
         case 2: hipLaunchKernelGGL((fetch_small_kernel<T, IdxT, 2>), dim3(grid), dim3(block), 0, stream, p); break;
-        default: hipLaunchKernelGGL((fetch_small_kernel<T, IdxT, 4>), dim3(grid), dim3(block), 0, stream, p); break;
+        default: hipLaunchKernelGGL((fetch_small_kernel<T, IdxT, 8>), dim3(grid), dim3(block), 0, stream, p); break;
         }
     } else {
         hipLaunchKernelGGL((fetch_large_kernel<T, IdxT>), dim3(grid), dim3(block), 0, stream, p);
@@ -111,9 +112,10 @@ int fetch_launch(const FetchParams &p, int elem_bytes, int index_bytes, int impl
     const DeviceInfo &dev = device_info();
     // Blocks are at most 256 threads (four waves); larger requests are capped.
     const int block = (nthreads > 0 && nthreads <= 256) ? nthreads : 256;
-    int64_t waves_needed = p.vec_size <= kWave ? (p.n + 3) / 4 : p.n;
+    const int rows = g_rows_in_flight.load();
+    int64_t waves_needed = p.vec_size <= kWave ? (p.n + rows - 1) / rows : p.n;
     int64_t grid = nblks > 0 ? nblks : (waves_needed * kWave + block - 1) / block;
-    const int64_t cap = (int64_t)dev.num_cus * 8;
+    const int64_t cap = (int64_t)dev.num_cus * g_blocks_per_cu.load();
     if (nblks <= 0 && grid > cap) grid = cap;
     if (grid < 1) grid = 1;
     const bool idx64 = index_bytes == 8;
@@ -171,4 +173,14 @@ extern "C" int ibp_decompress_fetch(int elem_bytes, void *out, const void *comp,
     p.err = index ? index_error_counter() : nullptr;
     const int cl = comp_len > 0 ? comp_len : (int)vec_size;
     return fetch_launch(p, elem_bytes, index_bytes, impl, cl, nblks, nthreads, as_stream(stream));
+}
+
+extern "C" int ibp_set_fetch_tuning(int rows_in_flight, int blocks_per_cu)
+{
+    IBP_CHECK(rows_in_flight == 2 || rows_in_flight == 4 || rows_in_flight == 8 || rows_in_flight == 16,
+              "rows_in_flight must be 2, 4, 8 or 16");
+    IBP_CHECK(blocks_per_cu >= 1 && blocks_per_cu <= 32, "blocks_per_cu must be in [1, 32]");
+    g_rows_in_flight.store(rows_in_flight);
+    g_blocks_per_cu.store(blocks_per_cu);
+    return IBP_OK;
 }

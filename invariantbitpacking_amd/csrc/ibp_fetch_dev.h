@@ -41,13 +41,22 @@ __device__ __forceinline__ int64_t load_index(const void *index, int64_t i)
     return index ? (int64_t)((const IdxT *)index)[i] : i;
 }
 
-// Column constants of lane j for rows of at most 64 elements.
+// Column constants of lane j for rows of at most 64 elements.  The zero runs
+// of the column mask are tabulated once per kernel: a flagged element's free
+// bits (top-aligned in v) land in run k as (v >> shift[k]) & run[k], where
+// shift[k] counts the mask bits above run k.  Masks with more than kRunCap
+// runs in some lane of the wave fall back to the generic deposit loop.
+constexpr int kRunCap = 4;
+
 template <typename T>
 struct SmallCols {
     T bitval;
     T free; // ~mask: positions carried in the payload of a flagged element
     int len_flagged; // B - popc(mask)
     bool valid;      // j < V
+    int wave_runs;   // max zero runs over the wave's lanes (uniform)
+    int shift[kRunCap];
+    T run[kRunCap];
 };
 
 template <typename T>
@@ -58,9 +67,47 @@ __device__ __forceinline__ SmallCols<T> load_small_cols(const void *mask, const 
     c.valid = lane < V;
     const T m = c.valid ? ((const T *)mask)[lane] : T(0);
     c.bitval = c.valid ? ((const T *)bitval)[lane] : T(0);
-    c.free = T(~m);
+    c.free = c.valid ? T(~m) : T(0);
     c.len_flagged = Bits<T>::B - popc(m);
+    T z = c.free;
+    int filled = 0, runs = 0;
+#pragma unroll
+    for (int k = 0; k < kRunCap; ++k) {
+        c.shift[k] = 0;
+        c.run[k] = T(0);
+    }
+    while (z) {
+        const int sft = clz(z);
+        const int r = clz(T(~T(z << sft)));
+        const T run = shr(himask<T>(r), sft);
+#pragma unroll
+        for (int k = 0; k < kRunCap; ++k)
+            if (k == runs) {
+                c.shift[k] = sft - filled;
+                c.run[k] = run;
+            }
+        filled += r;
+        z &= ~run;
+        ++runs;
+    }
+    int wr = 0;
+    for (int k = 1; k <= kRunCap + 1; ++k)
+        if (ballot(runs >= k)) wr = k;
+    c.wave_runs = wr;
     return c;
+}
+
+// Free bits of a flagged element (top-aligned in v) dropped into the zero
+// positions of the column mask.
+template <typename T>
+__device__ __forceinline__ T small_deposit(const SmallCols<T> &c, T v)
+{
+    if (c.wave_runs > kRunCap) return deposit_msb<T>(v, c.free);
+    T d = T(0);
+#pragma unroll
+    for (int k = 0; k < kRunCap; ++k)
+        if (k < c.wave_runs) d |= T(v >> c.shift[k]) & c.run[k];
+    return d;
 }
 
 // Header flag of lane `lane` for a row of <= 64 elements, given the first
@@ -78,92 +125,100 @@ __device__ __forceinline__ bool small_flag(T w, int lane)
     }
 }
 
-// Fetches and unpacks ROWS consecutive output rows [base, base + ROWS) of a
-// table whose rows have at most 64 elements.  All loads of one stage are
-// issued for every row before any of them is consumed.
+// Fetches and unpacks rows of a table whose rows have at most 64 elements:
+// groups of ROWS consecutive output rows starting at begin, begin + step, ...
+// below end.  Per group, every row's slot read is issued at once (a prefix of
+// first_words words, or the whole raw row) in parallel with the bitmask words,
+// and the next group's indices are prefetched, so a wave keeps ROWS rows and
+// one index load in flight.  Wave-uniform.
 template <typename T, typename IdxT, int ROWS>
-__device__ __forceinline__ void fetch_small_group(const FetchParams &p, const SmallCols<T> &col, int64_t base)
+__device__ __forceinline__ void fetch_small_range(const FetchParams &p, const SmallCols<T> &col, int64_t begin,
+                                                  int64_t end, int64_t step)
 {
     constexpr int B = Bits<T>::B;
     const int lane = lane_id();
     const int V = (int)p.vec_size;
     const int Hw = (int)(header_bytes(V, (int)sizeof(T)) / (int)sizeof(T));
+    const int E = p.first_words;
     const T *comp = (const T *)p.comp;
     T *out = (T *)p.out;
+    const bool has_index = p.index != nullptr;
 
-    // Stage 1: indices of the group (one load), then bitmask words (one gather).
-    int64_t my_src = 0;
-    if (lane < ROWS && base + lane < p.n) my_src = load_index<IdxT>(p.index, base + lane);
-    const bool my_ok = lane < ROWS && base + lane < p.n && my_src >= 0 && my_src < p.num_vecs;
-    uint32_t my_bm = 0;
-    if (my_ok) my_bm = p.bitmask[my_src >> 5];
-    const uint64_t okmask = ballot(my_ok);
-    const uint64_t pkmask = ballot(my_ok && ((my_bm >> (my_src & 31)) & 1u));
+    int64_t nxt = 0;
+    if (has_index && lane < ROWS && begin + lane < end) nxt = load_index<IdxT>(p.index, begin + lane);
+    for (int64_t base = begin; base < end; base += step) {
+        const bool live = lane < ROWS && base + lane < end;
+        const int64_t cur = has_index ? nxt : base + lane;
+        const bool my_ok = live && cur >= 0 && cur < p.num_vecs;
+        const uint64_t okmask = ballot(my_ok);
 
-    int64_t src[ROWS];
-    int need[ROWS];
-    T w[ROWS];
+        // Stage 1: slot prefixes of every row, bitmask words, next indices.
+        int64_t src[ROWS];
+        T w[ROWS];
 #pragma unroll
-    for (int r = 0; r < ROWS; ++r) {
-        src[r] = readlane(my_src, r);
-        const bool ok = (okmask >> r) & 1ull;
-        const bool packed = (pkmask >> r) & 1ull;
-        need[r] = ok ? (packed ? p.first_words : V) : 0;
-    }
-    // Stage 2: first read of every row (whole raw rows, speculative prefix of
-    // packed ones).
-#pragma unroll
-    for (int r = 0; r < ROWS; ++r) {
-        w[r] = T(0);
-        if (lane < need[r]) w[r] = comp[src[r] * V + lane];
-    }
-    // Stage 3: header -> flags -> bit offsets; second read where the prefix was short.
-    int pos[ROWS];
-    bool flag[ROWS];
-#pragma unroll
-    for (int r = 0; r < ROWS; ++r) {
-        const bool packed = (pkmask >> r) & 1ull;
-        flag[r] = false;
-        pos[r] = 0;
-        if (packed) { // wave-uniform
-            flag[r] = col.valid && small_flag<T>(w[r], lane);
-            const int L = col.valid ? (flag[r] ? col.len_flagged : B) : 0;
-            const int incl = wave_incl_scan(L);
-            const int total = readlane(incl, kWave - 1);
-            pos[r] = incl - L;
-            // Clamped to the slot: a corrupt header cannot make the read leave the row.
-            const int nw = Hw + (total + B - 1) / B;
-            need[r] = nw < V ? nw : V;
+        for (int r = 0; r < ROWS; ++r) {
+            src[r] = readlane(cur, r);
+            w[r] = T(0);
+            if (((okmask >> r) & 1ull) && lane < E) w[r] = comp[src[r] * V + lane];
         }
-    }
-#pragma unroll
-    for (int r = 0; r < ROWS; ++r) {
-        const bool packed = (pkmask >> r) & 1ull;
-        if (packed && need[r] > p.first_words && lane >= p.first_words && lane < need[r])
-            w[r] = comp[src[r] * V + lane];
-    }
-    // Stage 4: unpack and store.
-#pragma unroll
-    for (int r = 0; r < ROWS; ++r) {
-        const int64_t i = base + r;
-        const bool ok = (okmask >> r) & 1ull;
-        const bool packed = (pkmask >> r) & 1ull;
-        T val = w[r];
-        if (packed) {
-            const int q = Hw + pos[r] / B;
-            const int o = pos[r] % B;
-            const T w0 = shfl(w[r], q);
-            const T w1 = shfl(w[r], q + 1 < kWave ? q + 1 : kWave - 1);
-            const T v = stream_bits<T>(w0, w1, o);
-            val = flag[r] ? T(col.bitval | deposit_msb<T>(v, col.free)) : v;
+        uint32_t my_bm = 0;
+        if (my_ok) my_bm = p.bitmask[cur >> 5];
+        if (has_index) {
+            const int64_t nb = base + step;
+            nxt = 0;
+            if (lane < ROWS && nb + lane < end) nxt = load_index<IdxT>(p.index, nb + lane);
         }
-        if (i < p.n && lane < V) out[i * V + lane] = ok ? val : T(0);
-    }
-    // The ballot needs the whole wave active, so it is taken before the lane-0 branch.
-    const uint64_t live = ballot(lane < ROWS && base + lane < p.n);
-    if (p.err && lane == 0) {
-        const int bad = __builtin_popcountll(live & ~okmask);
-        if (bad) atomicAdd(p.err, (unsigned long long)bad);
+        const uint64_t pkmask = ballot(my_ok && ((my_bm >> (cur & 31)) & 1u));
+
+        // Stage 2: header -> flags -> bit offsets -> words needed; read the rest.
+        int pos[ROWS];
+        bool flag[ROWS];
+        int need[ROWS];
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r) {
+            const bool ok = (okmask >> r) & 1ull;
+            const bool packed = (pkmask >> r) & 1ull;
+            flag[r] = false;
+            pos[r] = 0;
+            need[r] = ok ? V : 0;
+            if (packed) { // wave-uniform
+                flag[r] = col.valid && small_flag<T>(w[r], lane);
+                const int L = col.valid ? (flag[r] ? col.len_flagged : B) : 0;
+                const int incl = wave_incl_scan(L);
+                const int total = readlane(incl, kWave - 1);
+                pos[r] = incl - L;
+                // Clamped to the slot: a corrupt header cannot make the read leave the row.
+                const int nw = Hw + (total + B - 1) / B;
+                need[r] = nw < V ? nw : V;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r)
+            if (need[r] > E && lane >= E && lane < need[r]) w[r] = comp[src[r] * V + lane];
+
+        // Stage 3: unpack and store.
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r) {
+            const int64_t i = base + r;
+            const bool ok = (okmask >> r) & 1ull;
+            const bool packed = (pkmask >> r) & 1ull;
+            T val = w[r];
+            if (packed) {
+                const int q = Hw + pos[r] / B;
+                const int o = pos[r] % B;
+                const T w0 = shfl(w[r], q);
+                const T w1 = shfl(w[r], q + 1 < kWave ? q + 1 : kWave - 1);
+                const T v = stream_bits<T>(w0, w1, o);
+                val = flag[r] ? T(col.bitval | small_deposit<T>(col, v)) : v;
+            }
+            if (i < end && lane < V) out[i * V + lane] = ok ? val : T(0);
+        }
+        // The ballot needs the whole wave active, so it is taken before the lane-0 branch.
+        const uint64_t livemask = ballot(live);
+        if (p.err && lane == 0) {
+            const int bad = __builtin_popcountll(livemask & ~okmask);
+            if (bad) atomicAdd(p.err, (unsigned long long)bad);
+        }
     }
 }
 

@@ -124,7 +124,7 @@ struct ibp_server {
     char *arena_mem = nullptr;
     Arena arena;
     size_t budget_override = 0;
-    BatchConfig bcfg{32, 2, 4, 0};
+    BatchConfig bcfg{128, 2, 8, 0, 4};
     std::vector<Set> sets;
     std::vector<Slot> slots;
     std::unordered_map<int64_t, Req> reqs;
