@@ -11,7 +11,7 @@ the compressed tensor resident in HBM (`value`), the fetch kernel's roofline
 (algorithmic HBM bytes / kernel time, against 8 TB/s), a CPU baseline (the
 oracle's single-threaded unpack of a bounded sample), and side figures: the
 batched form (all minibatches in one persistent launch), the PCIe-resident
-fetch against a measured pinned H2D copy ceiling, and p50/p99 call latency.
+fetch against a measured pinned H2D copy ceiling, and call latency.
 """
 import argparse
 import json
@@ -33,25 +33,39 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def timed_calls(fn, steps, warmup):
-    """Runs fn() warmup + steps times on the current stream; returns per-call
-    milliseconds (HIP events around each call) and the wall time of the timed
-    region (synchronised on both sides)."""
+def timed_region(fn, steps, warmup):
+    """warmup calls, then `steps` calls bracketed by a synchronisation and HIP
+    events on the current stream (the stream the kernels run on).  Returns the
+    mean milliseconds per call from the events and the wall seconds of the
+    region."""
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
-    for s, e in ev:
-        s.record()
+    s.record()
+    for _ in range(steps):
         fn()
-        e.record()
+    e.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    return [s.elapsed_time(e) for s, e in ev], wall
+    return s.elapsed_time(e) / steps, wall
+
+
+def call_latencies(fn, calls):
+    """Wall milliseconds of isolated calls (call issued -> results ready)."""
+    out = []
+    for _ in range(calls):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        out.append((time.perf_counter() - t0) * 1e3)
+    return out
 
 
 def h2d_ceiling_gbps(nbytes=256 << 20, reps=5):
+    """Pinned host -> device copy of 256 MiB (the PCIe ceiling of this box)."""
     src = torch.empty(nbytes, dtype=torch.uint8).pin_memory()
     dst = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
     dst.copy_(src, non_blocking=True)
@@ -91,6 +105,49 @@ def algorithmic_hbm_bytes(sizes_sel, n, v, w):
     return int(sizes_sel) + n * v * w + n * 8 + (n + 31) // 32 * 4
 
 
+def run_batched(ibp, comp, mask, bitval, bitmask, comp_len, idx, x, batch, v):
+    """All minibatches of the list in ONE persistent launch (autotuned,
+    replayed from a HIP graph)."""
+    dec_bytes = batch * v * 8
+    outs = [torch.empty((batch, v), dtype=torch.int64, device="cuda") for _ in idx]
+    bf = ibp.BatchedFetch([dict(comp=comp, mask=mask, bitval=bitval, bitmask=bitmask, index=i, out=o,
+                                comp_len=comp_len) for i, o in zip(idx, outs)])
+    bf.autotune()
+    bf.capture_graph()
+    bms, _ = timed_region(bf.graph_launch, 10, 2)
+    batched = len(idx) * dec_bytes / (bms / 1e3) / 1e9
+    exact = all(bool(torch.equal(o, x[i])) for o, i in zip(outs, idx))
+    desc = bf.describe()
+    log("batched: %d minibatches/launch %.3f ms, %.1f GB/s (%s)" % (len(idx), bms, batched, desc))
+    bf.close()
+    return {"decompressed_gbps": round(batched, 2), "minibatches_per_launch": len(idx),
+            "ms_per_launch": round(bms, 4), "bitexact": exact, "plan": json.loads(desc)}
+
+
+def run_pcie(ibp, comp, mask, bitval, bitmask, comp_len, idx, x, out, sel, steps):
+    """The same fetch with the compressed table in pinned host memory: the
+    kernel reads it over PCIe.  Compared with a pinned H2D copy."""
+    comp_h = comp.cpu().pin_memory()
+    ceiling = h2d_ceiling_gbps()
+    step = [0]
+
+    def fetch_h():
+        ibp.decompress_fetch(comp_h, mask, bitval, bitmask, torch.device("cuda"), comp_len, idx[step[0] % len(idx)],
+                             None, None, None, out)
+        step[0] += 1
+    hk, _ = timed_region(fetch_h, steps, 1)
+    exact = bool(torch.equal(out, x[idx[(step[0] - 1) % len(idx)]]))
+    pcie_bytes = float(np.mean(sel))
+    dec = out.numel() * out.element_size()
+    res = {"decompressed_gbps": round(dec / (hk / 1e3) / 1e9, 2),
+           "compressed_pcie_gbps": round(pcie_bytes / (hk / 1e3) / 1e9, 2),
+           "h2d_copy_ceiling_gbps": round(ceiling, 2),
+           "frac_of_ceiling": round(pcie_bytes / (hk / 1e3) / 1e9 / ceiling, 3),
+           "ms_per_call": round(hk, 4), "bitexact": exact}
+    log("pcie: %s" % res)
+    return res
+
+
 def run_gnn_minibatch(args):
     import invariantbitpacking_amd as ibp
     from invariantbitpacking_amd import workloads
@@ -118,59 +175,36 @@ def run_gnn_minibatch(args):
     step = [0]
 
     def fetch():
-        ibp.decompress_fetch(comp, mask, bitval, bitmask, torch.device("cuda"), comp_len,
-                             idx[step[0] % len(idx)], None, None, None, out)
+        ibp.decompress_fetch(comp, mask, bitval, bitmask, torch.device("cuda"), comp_len, idx[step[0] % len(idx)],
+                             None, None, None, out)
         step[0] += 1
 
-    # -- plain per-call fetch, compressed data resident in HBM (the metric)
-    ms, wall = timed_calls(fetch, args.steps, args.warmup)
-    last = idx[(step[0] - 1) % len(idx)]
-    bitexact = bool(torch.equal(out, x[last]))
+    # -- plain per-call fetch, compressed table resident in HBM (the metric)
+    kern_ms, wall = timed_region(fetch, args.steps, args.warmup)
+    bitexact = bool(torch.equal(out, x[idx[(step[0] - 1) % len(idx)]]))
     dec_bytes = batch * v * w
     value = args.steps * dec_bytes / wall / 1e9
-    kern_ms = float(np.mean(ms))
+    lat = call_latencies(fetch, max(20, args.steps))
     sel = [int(sizes[i].sum()) for i in idx]
     traffic_alg = algorithmic_hbm_bytes(np.mean(sel), batch, v, w)
     achieved = traffic_alg / (kern_ms / 1e3) / 1e9
-    log("plain: %.3f ms/call (events), %.1f GB/s decompressed, kernel-side %.1f GB/s HBM" % (kern_ms, value, achieved))
+    log("plain: %.4f ms/call (events), %.1f GB/s decompressed, kernel-side %.1f GB/s HBM" % (kern_ms, value, achieved))
 
-    # -- batched: every minibatch of the step list in one persistent launch
-    outs = [torch.empty((batch, v), dtype=torch.int64, device="cuda") for _ in idx]
-    bf = ibp.BatchedFetch([dict(comp=comp, mask=mask, bitval=bitval, bitmask=bitmask, index=i, out=o,
-                                comp_len=comp_len) for i, o in zip(idx, outs)])
-    bf.autotune()
-    bf.capture_graph()
-    bms, bwall = timed_calls(bf.graph_launch, max(3, args.steps // len(idx)), 1)
-    batched = len(idx) * dec_bytes / (float(np.mean(bms)) / 1e3) / 1e9
-    b_exact = all(bool(torch.equal(o, x[i])) for o, i in zip(outs, idx))
-    log("batched: %d minibatches/launch %.3f ms, %.1f GB/s (%s)" % (len(idx), float(np.mean(bms)), batched,
-                                                                    bf.describe()))
-    bf.close()
-    del outs
+    if args.calibrate_copy:
+        a = torch.empty(256 << 20, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(4):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
 
-    # -- PCIe-resident: compressed data in pinned host memory
+    batched = None
+    if not args.no_batched:
+        batched = run_batched(ibp, comp, mask, bitval, bitmask, comp_len, idx, x, batch, v)
+        batched["vs_plain"] = round(batched["decompressed_gbps"] / value, 3)
     pcie = None
     if not args.no_pcie:
-        comp_h = comp.cpu().pin_memory()
-        ceiling = h2d_ceiling_gbps()
-
-        def fetch_h():
-            ibp.decompress_fetch(comp_h, mask, bitval, bitmask, torch.device("cuda"), comp_len,
-                                 idx[step[0] % len(idx)], None, None, None, out)
-            step[0] += 1
-        hms, hwall = timed_calls(fetch_h, max(3, args.steps // 2), 1)
-        hk = float(np.mean(hms))
-        pcie_bytes = float(np.mean(sel))
-        pcie = {"decompressed_gbps": round(dec_bytes / (hk / 1e3) / 1e9, 2),
-                "compressed_pcie_gbps": round(pcie_bytes / (hk / 1e3) / 1e9, 2),
-                "h2d_copy_ceiling_gbps": round(ceiling, 2),
-                "frac_of_ceiling": round(pcie_bytes / (hk / 1e3) / 1e9 / ceiling, 3),
-                "ms_per_call": round(hk, 4),
-                "bitexact": bool(torch.equal(out, x[idx[(step[0] - 1) % len(idx)]]))}
-        log("pcie: %s" % pcie)
-        del comp_h
-
-    # -- CPU baseline (oracle, one core)
+        pcie = run_pcie(ibp, comp, mask, bitval, bitmask, comp_len, idx, x, out, sel, max(3, args.steps // 2))
     cpu = None
     if not args.no_cpu:
         comp_np = comp.cpu().numpy()
@@ -180,7 +214,7 @@ def run_gnn_minibatch(args):
         cpu = cpu_baseline(comp_np, om, ob, obm, [i.numpy() for i in idx_cpu], budget_s=args.cpu_seconds)
         log("cpu baseline: %s" % cpu)
 
-    res = {
+    return {
         "metric": METRIC, "value": round(value, 2), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "int64", "data": "synthetic",
@@ -191,13 +225,136 @@ def run_gnn_minibatch(args):
                      "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
                      "algorithmic_bytes_per_launch": traffic_alg, "kernel_ms": round(kern_ms, 5)},
         "cpu_baseline": cpu,
-        "bitexact": bitexact and b_exact,
-        "latency_ms": {"p50": round(float(np.percentile(ms, 50)), 4), "p99": round(float(np.percentile(ms, 99)), 4)},
-        "batched": {"decompressed_gbps": round(batched, 2), "minibatches_per_launch": len(idx),
-                    "vs_plain": round(batched / value, 3)},
+        "bitexact": bitexact and (batched is None or batched["bitexact"]) and (pcie is None or pcie["bitexact"]),
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4),
+                       "what": "isolated call: decompress_fetch issued -> rows ready (host wall clock)"},
+        "batched": batched,
         "pcie_resident": pcie,
     }
+
+
+def _compress_set(ibp, x):
+    mask, bitval = ibp.preprocess(x, None)
+    sizes = ibp.get_compress_size(x, mask, bitval)
+    comp = x.clone()
+    bitmask = ibp.compress_inplace(comp, mask, bitval)
+    comp_len = ibp.get_single_compress_len(x, mask, bitval)
+    return dict(x=x, mask=mask, bitval=bitval, sizes=sizes, comp=comp, bitmask=bitmask, comp_len=comp_len)
+
+
+def run_dlrm(args):
+    """26 embedding tables (rows log-uniform in [1e3, 1e7], dim 128 fp32 as
+    int64 [rows x 64]), each preprocessed separately; a batch gathers 65,536
+    Zipf(1.05) rows per table.  Plain = 26 decompress_fetch calls; batched =
+    one persistent launch (autotuned, HIP graph) for all 26 tables."""
+    import invariantbitpacking_amd as ibp
+    from invariantbitpacking_amd import workloads
+    tables = workloads.dlrm_tables(scale=args.scale, device="cuda")
+    sets = [_compress_set(ibp, x) for x in tables]
+    torch.cuda.synchronize()
+    rows = [x.shape[0] for x in tables]
+    idx = [i.cuda() for i in workloads.dlrm_indices(rows, batch=args.batch)]
+    outs = [torch.empty((args.batch, 64), dtype=torch.int64, device="cuda") for _ in tables]
+    comp_bytes = sum(int(s["sizes"].sum()) for s in sets)
+    raw_bytes = sum(r * 512 for r in rows)
+    log("dlrm: %d tables, %d rows, ratio %.3f" % (len(tables), sum(rows), raw_bytes / comp_bytes))
+
+    def plain():
+        for s, i, o in zip(sets, idx, outs):
+            ibp.decompress_fetch(s["comp"], s["mask"], s["bitval"], s["bitmask"], torch.device("cuda"),
+                                 s["comp_len"], i, None, None, None, o)
+    p_ms, p_wall = timed_region(plain, args.steps, args.warmup)
+    exact = all(bool(torch.equal(o, s["x"][i])) for s, i, o in zip(sets, idx, outs))
+    dec = len(tables) * args.batch * 512
+    bf = ibp.BatchedFetch([dict(comp=s["comp"], mask=s["mask"], bitval=s["bitval"], bitmask=s["bitmask"], index=i,
+                                out=o, comp_len=s["comp_len"]) for s, i, o in zip(sets, idx, outs)])
+    bf.autotune()
+    bf.capture_graph()
+    for o in outs:
+        o.fill_(0)
+    b_ms, _ = timed_region(bf.graph_launch, args.steps, args.warmup)
+    b_exact = all(bool(torch.equal(o, s["x"][i])) for s, i, o in zip(sets, idx, outs))
+    sel = sum(int(s["sizes"][i].sum()) for s, i in zip(sets, idx))
+    alg = sel + dec + len(tables) * (args.batch * 8 + (args.batch + 31) // 32 * 4)
+    res = {"workload": "dlrm: 26 tables, %d rows total, 65536 Zipf(1.05) rows per table per batch" % sum(rows),
+           "compression_ratio": round(raw_bytes / comp_bytes, 3),
+           "plain_ms_per_batch": round(p_ms, 4), "plain_gbps": round(dec / (p_ms / 1e3) / 1e9, 1),
+           "batched_ms_per_batch": round(b_ms, 4), "batched_gbps": round(dec / (b_ms / 1e3) / 1e9, 1),
+           "batched_vs_plain": round(p_ms / b_ms, 3), "batched_hbm_alg_gbps": round(alg / (b_ms / 1e3) / 1e9, 1),
+           "plan": json.loads(bf.describe()), "bitexact": exact and b_exact}
+    bf.close()
     return res
+
+
+def run_serve(args):
+    """Requests drawn from three resident sets (GNN features, GNN minibatch
+    table, one DLRM table), index counts log-uniform in [1e3, 2.5e5], Poisson
+    arrivals at each rate.  Served: continuous batching over 4 streams.
+    Plain: one decompress_fetch per request in arrival order."""
+    import invariantbitpacking_amd as ibp
+    from invariantbitpacking_amd import workloads
+    d1, _ = workloads.gnn_features()
+    s1 = _compress_set(ibp, d1.cuda())
+    s2 = _compress_set(ibp, workloads.gnn_minibatch(scale=args.scale, device="cuda"))
+    s3 = _compress_set(ibp, workloads.dlrm_tables(num_tables=1, scale=args.scale * 8, device="cuda")[0])
+    sets = [s1, s2, s3]
+    torch.cuda.synchronize()
+    srv = ibp.Server(hbm_budget_bytes=8 << 30, num_streams=4, max_batch_requests=64, max_batch_rows=1 << 21)
+    sids = [srv.register(s["comp"], s["mask"], s["bitval"], s["bitmask"], s["comp_len"]) for s in sets]
+    rng = np.random.default_rng(300)
+    results = []
+    for rate in args.rates:
+        nreq = int(min(400, max(50, rate * 0.5)))
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, size=nreq))
+        reqs = []
+        for _ in range(nreq):
+            k = int(rng.integers(0, 3))
+            n = int(np.exp(rng.uniform(np.log(1e3), np.log(2.5e5))))
+            reqs.append((k, rng.integers(0, sets[k]["x"].shape[0], size=n)))
+        dec = sum(len(i) * sets[k]["x"].shape[1] * 8 for k, i in reqs)
+        # served
+        t0 = time.perf_counter()
+        handles = []
+        for (k, i), ta in zip(reqs, arrivals):
+            while time.perf_counter() - t0 < ta:
+                pass
+            handles.append(srv.submit(sids[k], i))
+        lat = []
+        ok = True
+        for (k, i), h in zip(reqs, handles):
+            out, us = srv.wait(h)
+            lat.append(us / 1e3)
+            if len(lat) <= 8:
+                ok = ok and bool(torch.equal(out, sets[k]["x"][torch.from_numpy(i).cuda()]))
+            srv.release(h)
+        served_wall = time.perf_counter() - t0
+        # plain per-request calls
+        outs = []
+        plat = []
+        t0 = time.perf_counter()
+        for (k, i), ta in zip(reqs, arrivals):
+            while time.perf_counter() - t0 < ta:
+                pass
+            s = sets[k]
+            o = ibp.decompress_fetch(s["comp"], s["mask"], s["bitval"], s["bitmask"], torch.device("cuda"),
+                                     s["comp_len"], torch.from_numpy(i).cuda())
+            torch.cuda.synchronize()
+            plat.append((time.perf_counter() - t0 - ta) * 1e3)
+        plain_wall = time.perf_counter() - t0
+        r = {"rate_per_s": rate, "requests": nreq,
+             "served": {"p50_ms": round(float(np.percentile(lat, 50)), 3),
+                        "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                        "gbps": round(dec / served_wall / 1e9, 2)},
+             "plain": {"p50_ms": round(float(np.percentile(plat, 50)), 3),
+                       "p99_ms": round(float(np.percentile(plat, 99)), 3),
+                       "gbps": round(dec / plain_wall / 1e9, 2)},
+             "bitexact_sampled": ok}
+        log("serve rate %g: %s" % (rate, r))
+        results.append(r)
+    st = srv.stats()
+    srv.close()
+    return {"workload": "serving: Poisson arrivals over 3 resident sets", "rates": results,
+            "server_stats": {k: (round(v, 2) if isinstance(v, float) else v) for k, v in st.items()}}
 
 
 def main():
@@ -210,6 +367,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-pcie", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--workload", default="gnn_minibatch", choices=["gnn_minibatch", "dlrm", "serve"],
+                    help="gnn_minibatch is the headline line; the others print their own JSON")
+    ap.add_argument("--rates", type=float, nargs="+", default=[1e2, 1e3, 1e4])
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run 4 device copies of 256 MiB (the reference kernel for HBM counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -218,8 +381,14 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
-    res = run_gnn_minibatch(args)
-    print(json.dumps(res))
+    if args.workload == "dlrm":
+        if args.batch == 200000:
+            args.batch = 65536
+        print(json.dumps(run_dlrm(args)))
+    elif args.workload == "serve":
+        print(json.dumps(run_serve(args)))
+    else:
+        print(json.dumps(run_gnn_minibatch(args)))
 
 
 if __name__ == "__main__":

@@ -201,6 +201,10 @@ typedef struct ibp_server_stats {
     int64_t waited_admissions;
 } ibp_server_stats;
 int ibp_server_stats_get(ibp_server *srv, ibp_server_stats *st);
+/* Diagnostics: host steady-clock microseconds at which a request arrived,
+ * was admitted, entered the launch path, had its launch issued and was seen
+ * complete (times[0..4]; 0 = not reached).  Valid until ibp_server_release. */
+int ibp_server_request_times(ibp_server *srv, int64_t req_id, double *times);
 /* Test hook: pretend the arena has only `bytes` free (0 disables). */
 int ibp_server_set_budget_override(ibp_server *srv, size_t bytes);
 

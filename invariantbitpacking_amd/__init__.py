@@ -28,8 +28,11 @@ def _ptr(t):
     return None if t is None else ctypes.c_void_p(t.data_ptr())
 
 
-def _stream():
-    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+def _stream(device_index=None):
+    """Raw handle of the current HIP stream of the current (or given) device."""
+    if device_index is None:
+        device_index = torch.cuda.current_device()
+    return ctypes.c_void_p(torch._C._cuda_getCurrentRawStream(device_index))
 
 
 def _accessible(t):
@@ -241,7 +244,8 @@ def decompress_fetch(comp_dataset, mask, bitval, bitmask, device, comp_len=None,
     _check(bitmask.numel() >= (num_vecs + 31) // 32, "Bitmask too short for the dataset")
     idx, ib = _index_arg(index_arr, require_cuda=True)
     n = num_vecs if idx is None else idx.size(0)
-    device = torch.device(device)
+    if not isinstance(device, torch.device):
+        device = torch.device(device)
     cl = 0
     if comp_len is not None:
         cl = int(comp_len)

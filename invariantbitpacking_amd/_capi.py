@@ -63,6 +63,7 @@ SIGNATURES = {
     "ibp_server_release": (_I32, [_P, _I64]),
     "ibp_server_stats_get": (_I32, [_P, ctypes.POINTER(ServerStats)]),
     "ibp_server_set_budget_override": (_I32, [_P, _SZ]),
+    "ibp_server_request_times": (_I32, [_P, _I64, ctypes.POINTER(ctypes.c_double)]),
     "ibp_selftest_bitops": (_I32, [_I32, ctypes.c_uint64]),
 }
 

@@ -46,49 +46,91 @@ __device__ __forceinline__ void run_item(const TaskDev &t, int task, int64_t b, 
         }
         cached_task = task;
     } else {
-        for (int64_t i = b; i < e; ++i) {
-            if (t.elem_bytes == 8) {
-                if (i32) fetch_large_row<uint64_t, int32_t>(p, i);
-                else fetch_large_row<uint64_t, int64_t>(p, i);
-            } else {
-                if (i32) fetch_large_row<uint32_t, int32_t>(p, i);
-                else fetch_large_row<uint32_t, int64_t>(p, i);
-            }
+        if (t.elem_bytes == 8) {
+            if (i32) fetch_large_range<uint64_t, int32_t>(p, b, e, 1);
+            else fetch_large_range<uint64_t, int64_t>(p, b, e, 1);
+        } else {
+            if (i32) fetch_large_range<uint32_t, int32_t>(p, b, e, 1);
+            else fetch_large_range<uint32_t, int64_t>(p, b, e, 1);
         }
     }
 }
 
-// Persistent kernel: each wave claims `claim` consecutive work items per
-// atomic on the queue head until the queue is drained.
+// Tasks whose first work item is staged in LDS for the item -> task lookup.
+constexpr int kLdsTasks = 1024;
+
+// Persistent kernel over the work items of all tasks.  Static mode: wave w
+// takes items w, w + W, ... (no atomics; items are equal-sized except each
+// task's last).  Dynamic mode: waves claim `claim` consecutive items per
+// atomic on the queue head until it is drained (uneven item costs).  The
+// item -> task lookup is a binary search over first-item offsets staged in
+// LDS, skipped while consecutive items stay in the current task.
 template <int ROWS>
 __global__ __launch_bounds__(256) void batch_kernel(const TaskDev *tasks, int ntasks, int64_t total_items,
-                                                    unsigned long long *head, int claim)
+                                                    unsigned long long *head, int claim, int dynamic)
 {
+    __shared__ int64_t first_item[kLdsTasks];
     const int lane = lane_id();
+    const bool staged = ntasks <= kLdsTasks;
+    if (staged)
+        for (int k = threadIdx.x; k < ntasks; k += blockDim.x) first_item[k] = tasks[k].item_begin;
+    __syncthreads();
     int cached_task = -1;
+    int cur = -1;
+    int64_t cur_lo = 0, cur_hi = 0; // items [cur_lo, cur_hi) belong to task `cur`
+    TaskDev t;
     SmallCols<uint64_t> c64;
     SmallCols<uint32_t> c32;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
+    int64_t next = wave; // static mode: next item of this wave
     for (;;) {
-        unsigned long long claimed = 0;
-        if (lane == 0) claimed = atomicAdd(head, (unsigned long long)claim);
-        const int64_t first = readfirstlane((int64_t)claimed);
+        int64_t first, last;
+        if (dynamic) {
+            unsigned long long claimed = 0;
+            if (lane == 0) claimed = atomicAdd(head, (unsigned long long)claim);
+            first = readfirstlane((int64_t)claimed);
+            last = first + claim;
+        } else {
+            first = next;
+            last = next + 1;
+            next += nwaves;
+        }
         if (first >= total_items) break; // every wave reaches this exit
-        const int64_t last = first + claim < total_items ? first + claim : total_items;
+        if (last > total_items) last = total_items;
         for (int64_t item = first; item < last; ++item) {
-            // Task of the item: last task whose item_begin <= item.
-            int lo = 0, hi = ntasks - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) / 2;
-                if (tasks[mid].item_begin <= item) lo = mid;
-                else hi = mid - 1;
+            if (item < cur_lo || item >= cur_hi) {
+                int lo = 0, hi = ntasks - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) / 2;
+                    const int64_t fm = staged ? first_item[mid] : tasks[mid].item_begin;
+                    if (fm <= item) lo = mid;
+                    else hi = mid - 1;
+                }
+                cur = lo;
+                t = tasks[lo];
+                cur_lo = t.item_begin;
+                cur_hi = lo + 1 < ntasks ? (staged ? first_item[lo + 1] : tasks[lo + 1].item_begin) : total_items;
             }
-            const TaskDev t = tasks[lo];
             const int64_t b = (item - t.item_begin) * t.rows_per_item;
             int64_t e = b + t.rows_per_item;
             if (e > t.p.n) e = t.p.n;
-            run_item<ROWS>(t, lo, b, e, cached_task, c64, c32);
+            run_item<ROWS>(t, cur, b, e, cached_task, c64, c32);
         }
     }
+}
+
+BatchConfig adapt_config(const BatchConfig &base, int64_t total_rows, int num_cus)
+{
+    BatchConfig c = base;
+    const int64_t waves = base.blocks > 0 ? (int64_t)base.blocks * 4 : (int64_t)num_cus * 32;
+    int64_t rpi = total_rows / waves;
+    rpi = (rpi + c.rows_in_flight - 1) / c.rows_in_flight * c.rows_in_flight;
+    if (rpi < c.rows_in_flight) rpi = c.rows_in_flight;
+    if (rpi > base.rows_per_item_small) rpi = base.rows_per_item_small;
+    c.rows_per_item_small = (int)rpi;
+    if (total_rows < waves * 4) c.claim = 1;
+    return c;
 }
 
 int64_t fill_tasks(const ibp_fetch_task *tasks, int n, const BatchConfig &cfg, TaskDev *out)
@@ -123,12 +165,18 @@ int64_t fill_tasks(const ibp_fetch_task *tasks, int n, const BatchConfig &cfg, T
 int launch_batch(const TaskDev *dev_tasks, int ntasks, int64_t total_items, unsigned long long *head,
                  const BatchConfig &cfg, hipStream_t s)
 {
-    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, s, head);
-    const dim3 grid((unsigned)cfg.blocks), block(256);
+    // No more blocks than there is work for (claims of `claim` items in dynamic
+    // mode, one item per wave in static mode; four waves per block).
+    const int64_t per_wave = cfg.dynamic ? cfg.claim : 1;
+    int64_t blocks = (total_items + per_wave * 4 - 1) / (per_wave * 4);
+    if (blocks > cfg.blocks) blocks = cfg.blocks;
+    if (blocks < 1) blocks = 1;
+    const dim3 grid((unsigned)blocks), block(256);
+    if (cfg.dynamic) hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, s, head);
     switch (cfg.rows_in_flight) {
-    case 2: hipLaunchKernelGGL((batch_kernel<2>), grid, block, 0, s, dev_tasks, ntasks, total_items, head, cfg.claim); break;
-    case 8: hipLaunchKernelGGL((batch_kernel<8>), grid, block, 0, s, dev_tasks, ntasks, total_items, head, cfg.claim); break;
-    default: hipLaunchKernelGGL((batch_kernel<4>), grid, block, 0, s, dev_tasks, ntasks, total_items, head, cfg.claim); break;
+    case 2: hipLaunchKernelGGL((batch_kernel<2>), grid, block, 0, s, dev_tasks, ntasks, total_items, head, cfg.claim, cfg.dynamic); break;
+    case 8: hipLaunchKernelGGL((batch_kernel<8>), grid, block, 0, s, dev_tasks, ntasks, total_items, head, cfg.claim, cfg.dynamic); break;
+    default: hipLaunchKernelGGL((batch_kernel<4>), grid, block, 0, s, dev_tasks, ntasks, total_items, head, cfg.claim, cfg.dynamic); break;
     }
     IBP_LAUNCH_CHECK();
     return IBP_OK;
@@ -144,7 +192,7 @@ struct ibp_plan {
     TaskDev *dev_tasks = nullptr;
     unsigned long long *head = nullptr;
     int64_t total_items = 0;
-    BatchConfig cfg{128, 2, 8, 0, 4};
+    BatchConfig cfg{128, 2, 8, 0, 4, 0};
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     hipStream_t capture_stream = nullptr;
@@ -193,7 +241,7 @@ extern "C" int ibp_plan_create(const ibp_fetch_task *tasks, int num_tasks, ibp_p
     pl->tasks.assign(tasks, tasks + num_tasks);
     (void)hipGetDevice(&pl->device);
     const DeviceInfo &dev = device_info();
-    pl->cfg.blocks = dev.num_cus * 4;
+    pl->cfg.blocks = dev.num_cus * 8;
     if (hipMalloc((void **)&pl->dev_tasks, sizeof(TaskDev) * (size_t)num_tasks) != hipSuccess ||
         hipMalloc((void **)&pl->head, sizeof(unsigned long long)) != hipSuccess) {
         ibp_plan_destroy(pl);
@@ -232,11 +280,13 @@ extern "C" int ibp_plan_autotune(ibp_plan *pl, void *stream, float *best_ms)
     hipStream_t s = as_stream(stream);
     const DeviceInfo &dev = device_info();
     std::vector<BatchConfig> cands;
-    for (int rpi : {64, 128, 256})
+    for (int rpi : {32, 64, 128, 256})
         for (int rif : {4, 8})
-            for (int mult : {8, 16})
-                for (int claim : {1, 4, 16})
-                    cands.push_back(BatchConfig{rpi, rpi >= 128 ? 4 : 2, rif, dev.num_cus * mult, claim});
+            for (int mult : {8, 16}) {
+                cands.push_back(BatchConfig{rpi, rpi >= 128 ? 4 : 2, rif, dev.num_cus * mult, 1, 0});
+                for (int claim : {1, 8})
+                    cands.push_back(BatchConfig{rpi, rpi >= 128 ? 4 : 2, rif, dev.num_cus * mult, claim, 1});
+            }
     hipEvent_t e0, e1;
     IBP_HIP(hipEventCreate(&e0));
     IBP_HIP(hipEventCreate(&e1));
@@ -306,8 +356,8 @@ extern "C" int ibp_plan_describe(ibp_plan *pl, char *buf, size_t len)
     IBP_CHECK(pl != nullptr && buf != nullptr, "null argument");
     snprintf(buf, len,
              "{\"tasks\": %zu, \"items\": %lld, \"rows_per_item_small\": %d, \"rows_per_item_large\": %d, "
-             "\"rows_in_flight\": %d, \"blocks\": %d, \"claim\": %d, \"graph\": %s}",
+             "\"rows_in_flight\": %d, \"blocks\": %d, \"claim\": %d, \"dynamic\": %d, \"graph\": %s}",
              pl->tasks.size(), (long long)pl->total_items, pl->cfg.rows_per_item_small, pl->cfg.rows_per_item_large,
-             pl->cfg.rows_in_flight, pl->cfg.blocks, pl->cfg.claim, pl->graph_exec ? "true" : "false");
+             pl->cfg.rows_in_flight, pl->cfg.blocks, pl->cfg.claim, pl->cfg.dynamic, pl->graph_exec ? "true" : "false");
     return IBP_OK;
 }

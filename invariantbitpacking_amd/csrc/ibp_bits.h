@@ -82,10 +82,12 @@ __host__ __device__ __forceinline__ T deposit_msb(T v, T free)
 
 // The top B bits of the bit stream starting at bit o (0 <= o < B) of word w0,
 // continuing into w1: (w0 << o) | (w1 >> (B - o)).
+// (w1 >> 1) >> (B - 1 - o) equals w1 >> (B - o) for o in [1, B) and gives 0
+// for o = 0 without a range check.
 template <typename T>
 __host__ __device__ __forceinline__ T stream_bits(T w0, T w1, int o)
 {
-    return T(w0 << o) | shr(w1, Bits<T>::B - o);
+    return T(w0 << o) | T(T(w1 >> 1) >> (Bits<T>::B - 1 - o));
 }
 
 // Header and slot geometry (ibp_compress_dev.cuh:13-15; ibp_misc_dev.cuh:33-42).

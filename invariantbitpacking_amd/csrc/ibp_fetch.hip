@@ -29,7 +29,7 @@ __global__ __launch_bounds__(256) void fetch_large_kernel(FetchParams p)
 {
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
-    for (int64_t i = wave; i < p.n; i += nwaves) fetch_large_row<T, IdxT>(p, i);
+    fetch_large_range<T, IdxT>(p, wave, p.n, nwaves);
 }
 
 // Transfer baselines of the reference (include/decompress/ibp_decompress_host.cuh:114-166):

@@ -222,10 +222,10 @@ __device__ __forceinline__ void fetch_small_range(const FetchParams &p, const Sm
     }
 }
 
-// Fetches and unpacks output row i of a table with rows of any length, one
-// 64-element chunk at a time.  Wave-uniform.
-template <typename T, typename IdxT>
-__device__ __forceinline__ void fetch_large_row(const FetchParams &p, int64_t i)
+// Fetches and unpacks output row i (source row s, already loaded) of a table
+// with rows of any length, one 64-element chunk at a time.  Wave-uniform.
+template <typename T>
+__device__ __forceinline__ void fetch_large_row_from(const FetchParams &p, int64_t i, int64_t s)
 {
     constexpr int B = Bits<T>::B;
     constexpr int FLAGS_PER_WORD = B;
@@ -236,7 +236,6 @@ __device__ __forceinline__ void fetch_large_row(const FetchParams &p, int64_t i)
     const T *bitval = (const T *)p.bitval;
     T *dst = (T *)p.out + i * V;
 
-    const int64_t s = readfirstlane(load_index<IdxT>(p.index, i));
     if (s < 0 || s >= p.num_vecs) {
         for (int64_t k = lane; k < V; k += kWave) dst[k] = T(0);
         if (p.err && lane == 0) atomicAdd(p.err, 1ull);
@@ -308,6 +307,20 @@ __device__ __forceinline__ void fetch_large_row(const FetchParams &p, int64_t i)
         const T val = f ? T(b | deposit_msb<T>(v, T(~m))) : v;
         if (valid) dst[j] = val;
         cursor += tot;
+    }
+}
+
+// Output rows [b, e), step `step`, of a wide-row table.  The next row's
+// index is loaded while the current row is unpacked.
+template <typename T, typename IdxT>
+__device__ __forceinline__ void fetch_large_range(const FetchParams &p, int64_t b, int64_t e, int64_t step)
+{
+    if (b >= e) return;
+    int64_t s = readfirstlane(load_index<IdxT>(p.index, b));
+    for (int64_t i = b; i < e; i += step) {
+        const int64_t nxt = i + step < e ? readfirstlane(load_index<IdxT>(p.index, i + step)) : 0;
+        fetch_large_row_from<T>(p, i, s);
+        s = nxt;
     }
 }
 

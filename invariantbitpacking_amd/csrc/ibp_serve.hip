@@ -48,10 +48,12 @@ enum class St { Pending, Admitted, Running, Done };
 
 struct Req {
     int set = 0;
-    std::vector<int64_t> index;
+    std::vector<int64_t> index; // only while no pinned index space was free
     int64_t n = 0;
     size_t out_off = 0, out_bytes = 0;
-    double t_arrival = 0, t_done = 0;
+    size_t idx_off = 0;
+    bool idx_pinned = false;
+    double t_arrival = 0, t_admit = 0, t_launch = 0, t_issued = 0, t_done = 0;
     St st = St::Pending;
     bool waited = false;
 };
@@ -106,11 +108,13 @@ struct Arena {
     }
 };
 
+// One launch slot: a stream and the batch's task table, staged in pinned host
+// memory and copied to the device once per launch.  Request indices stay in
+// the server's pinned index arena and are read by the kernel in place.
 struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
-    TaskDev *dev_tasks = nullptr, *host_tasks = nullptr; // host: pinned staging
-    int64_t *dev_index = nullptr, *host_index = nullptr;
+    TaskDev *host_tasks = nullptr, *dev_tasks = nullptr;
     unsigned long long *head = nullptr;
     std::vector<int64_t> reqs;
     bool busy = false;
@@ -123,8 +127,10 @@ struct ibp_server {
     int device = 0;
     char *arena_mem = nullptr;
     Arena arena;
+    char *pin_mem = nullptr; // pinned host memory of request indices
+    Arena pin_arena;
     size_t budget_override = 0;
-    BatchConfig bcfg{128, 2, 8, 0, 4};
+    BatchConfig bcfg{256, 4, 8, 0, 1, 0};
     std::vector<Set> sets;
     std::vector<Slot> slots;
     std::unordered_map<int64_t, Req> reqs;
@@ -143,16 +149,29 @@ struct ibp_server {
         const size_t cap = budget_override ? std::min(budget_override, arena.size) : arena.size;
         while (!pending.empty()) {
             Req &r = reqs[pending.front()];
-            size_t off = 0;
-            if (!arena.alloc(r.out_bytes, cap, &off)) {
+            size_t off = 0, ioff = 0;
+            const bool need_pin = !r.idx_pinned;
+            bool ok = !need_pin || pin_arena.alloc(sizeof(int64_t) * (size_t)r.n, pin_arena.size, &ioff);
+            if (ok && !arena.alloc(r.out_bytes, cap, &off)) {
+                if (need_pin) pin_arena.release(ioff, sizeof(int64_t) * (size_t)r.n);
+                ok = false;
+            }
+            if (!ok) {
                 if (!r.waited) {
                     r.waited = true;
                     ++waited;
                 }
                 return; // FIFO: later requests wait behind the head
             }
+            if (need_pin) {
+                memcpy(pin_mem + ioff, r.index.data(), sizeof(int64_t) * (size_t)r.n);
+                std::vector<int64_t>().swap(r.index);
+                r.idx_off = ioff;
+                r.idx_pinned = true;
+            }
             r.out_off = off;
             r.st = St::Admitted;
+            r.t_admit = now_us();
             admitted.push_back(pending.front());
             pending.pop_front();
         }
@@ -160,6 +179,7 @@ struct ibp_server {
 
     bool launch_locked(Slot &sl)
     {
+        const double t_launch = now_us();
         std::vector<ibp_fetch_task> tasks;
         int64_t nrows = 0;
         sl.reqs.clear();
@@ -167,13 +187,12 @@ struct ibp_server {
             Req &r = reqs[admitted.front()];
             if (!sl.reqs.empty() && nrows + r.n > cfg.max_batch_rows) break;
             const Set &s = sets[r.set];
-            memcpy(sl.host_index + nrows, r.index.data(), sizeof(int64_t) * (size_t)r.n);
             ibp_fetch_task t;
             t.comp = s.comp;
             t.mask = s.mask;
             t.bitval = s.bitval;
             t.bitmask = s.bitmask;
-            t.index = sl.dev_index + nrows;
+            t.index = pin_mem + r.idx_off;
             t.out = arena_mem + r.out_off;
             t.num_vecs = s.num_vecs;
             t.vec_size = s.vec_size;
@@ -189,16 +208,19 @@ struct ibp_server {
             admitted.pop_front();
         }
         if (sl.reqs.empty()) return false;
-        const int64_t items = fill_tasks(tasks.data(), (int)tasks.size(), bcfg, sl.host_tasks);
-        hipError_t e = hipMemcpyAsync(sl.dev_index, sl.host_index, sizeof(int64_t) * (size_t)nrows,
-                                      hipMemcpyHostToDevice, sl.stream);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(sl.dev_tasks, sl.host_tasks, sizeof(TaskDev) * tasks.size(), hipMemcpyHostToDevice,
-                               sl.stream);
-        int st = e == hipSuccess ? launch_batch(sl.dev_tasks, (int)tasks.size(), items, sl.head, bcfg, sl.stream)
+        const BatchConfig cfg_now = adapt_config(bcfg, nrows, device_info().num_cus);
+        const int64_t items = fill_tasks(tasks.data(), (int)tasks.size(), cfg_now, sl.host_tasks);
+        const hipError_t e = hipMemcpyAsync(sl.dev_tasks, sl.host_tasks, sizeof(TaskDev) * tasks.size(),
+                                            hipMemcpyHostToDevice, sl.stream);
+        int st = e == hipSuccess ? launch_batch(sl.dev_tasks, (int)tasks.size(), items, sl.head, cfg_now, sl.stream)
                                  : IBP_ERR_HIP;
         if (st == IBP_OK && hipEventRecord(sl.done, sl.stream) != hipSuccess) st = IBP_ERR_HIP;
         if (st != IBP_OK) worker_error = "launch failed";
+        const double t_issued = now_us();
+        for (int64_t id : sl.reqs) {
+            reqs[id].t_launch = t_launch;
+            reqs[id].t_issued = t_issued;
+        }
         sl.busy = true;
         ++launches;
         launched_reqs += (int64_t)sl.reqs.size();
@@ -215,6 +237,8 @@ struct ibp_server {
             const double t = now_us();
             for (int64_t id : sl.reqs) {
                 Req &r = reqs[id];
+                pin_arena.release(r.idx_off, sizeof(int64_t) * (size_t)r.n);
+                r.idx_pinned = false;
                 r.st = St::Done;
                 r.t_done = t;
                 latencies.push_back(r.t_done - r.t_arrival);
@@ -257,30 +281,28 @@ extern "C" int ibp_server_create(const ibp_server_config *cfg, ibp_server **out)
     ibp_server *srv = new ibp_server();
     srv->cfg = *cfg;
     (void)hipGetDevice(&srv->device);
-    srv->bcfg.blocks = device_info().num_cus * 4;
+    srv->bcfg.blocks = device_info().num_cus * 8;
     auto fail = [&](const char *what, hipError_t e) {
         ibp_server_destroy(srv);
         return set_error(IBP_ERR_NOMEM, "server %s failed: %s", what, hipGetErrorString(e));
     };
     hipError_t e = hipMalloc((void **)&srv->arena_mem, cfg->hbm_budget_bytes);
     if (e != hipSuccess) return fail("arena allocation", e);
+    if ((e = hipDeviceSynchronize()) != hipSuccess) return fail("setup", e);
     srv->arena.init(cfg->hbm_budget_bytes);
+    size_t pin_bytes = sizeof(int64_t) * (size_t)cfg->max_batch_rows * 4;
+    if (pin_bytes < ((size_t)64 << 20)) pin_bytes = (size_t)64 << 20;
+    if ((e = hipHostMalloc((void **)&srv->pin_mem, pin_bytes, 0)) != hipSuccess) return fail("pinned index arena", e);
+    srv->pin_arena.init(pin_bytes);
     srv->slots.resize((size_t)cfg->num_streams);
     for (Slot &sl : srv->slots) {
         // Streams keep the default priority.
         if ((e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking)) != hipSuccess) return fail("stream", e);
         if ((e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming)) != hipSuccess) return fail("event", e);
-        if ((e = hipMalloc((void **)&sl.dev_tasks, sizeof(TaskDev) * (size_t)cfg->max_batch_requests)) != hipSuccess)
-            return fail("task table", e);
-        if ((e = hipHostMalloc((void **)&sl.host_tasks, sizeof(TaskDev) * (size_t)cfg->max_batch_requests, 0)) !=
-            hipSuccess)
-            return fail("task staging", e);
-        if ((e = hipMalloc((void **)&sl.dev_index, sizeof(int64_t) * (size_t)cfg->max_batch_rows)) != hipSuccess)
-            return fail("index buffer", e);
-        if ((e = hipHostMalloc((void **)&sl.host_index, sizeof(int64_t) * (size_t)cfg->max_batch_rows, 0)) !=
-            hipSuccess)
-            return fail("index staging", e);
-        if ((e = hipMalloc((void **)&sl.head, sizeof(unsigned long long))) != hipSuccess) return fail("queue head", e);
+        const size_t stage = sizeof(TaskDev) * (size_t)cfg->max_batch_requests;
+        if ((e = hipMalloc((void **)&sl.dev_tasks, stage)) != hipSuccess) return fail("task table", e);
+        if ((e = hipHostMalloc((void **)&sl.host_tasks, stage, 0)) != hipSuccess) return fail("task staging", e);
+        if ((e = hipMalloc((void **)&sl.head, sizeof(unsigned long long))) != hipSuccess) return fail("queue", e);
     }
     srv->worker = std::thread([srv] { srv->run(); });
     *out = srv;
@@ -302,11 +324,10 @@ extern "C" int ibp_server_destroy(ibp_server *srv)
         if (sl.done) (void)hipEventDestroy(sl.done);
         if (sl.dev_tasks) (void)hipFree(sl.dev_tasks);
         if (sl.host_tasks) (void)hipHostFree(sl.host_tasks);
-        if (sl.dev_index) (void)hipFree(sl.dev_index);
-        if (sl.host_index) (void)hipHostFree(sl.host_index);
         if (sl.head) (void)hipFree(sl.head);
     }
     if (srv->arena_mem) (void)hipFree(srv->arena_mem);
+    if (srv->pin_mem) (void)hipHostFree(srv->pin_mem);
     delete srv;
     return IBP_OK;
 }
@@ -326,6 +347,7 @@ extern "C" int ibp_server_register(ibp_server *srv, int elem_bytes, const void *
 
 extern "C" int ibp_server_submit(ibp_server *srv, int set_id, const int64_t *index, int64_t n, int64_t *req_id)
 {
+    const double t_arrival = now_us(); // latency includes copying the index
     IBP_CHECK(srv && req_id, "null argument");
     IBP_CHECK(n >= 1 && index != nullptr, "a request needs at least one index");
     std::unique_lock<std::mutex> lk(srv->mu);
@@ -338,12 +360,22 @@ extern "C" int ibp_server_submit(ibp_server *srv, int set_id, const int64_t *ind
         return set_error(IBP_ERR_NOMEM, "request output (%zu bytes) can never fit the arena (%zu bytes)", bytes,
                          srv->arena.size);
     const int64_t id = srv->next_id++;
+    size_t ioff = 0;
+    const bool pinned = srv->pin_arena.alloc(sizeof(int64_t) * (size_t)n, srv->pin_arena.size, &ioff);
+    lk.unlock();
+    // The copy runs outside the lock; the request is not queued until it is done.
+    std::vector<int64_t> vec;
+    if (pinned) memcpy(srv->pin_mem + ioff, index, sizeof(int64_t) * (size_t)n);
+    else vec.assign(index, index + n);
+    lk.lock();
     Req &r = srv->reqs[id];
     r.set = set_id;
-    r.index.assign(index, index + n);
+    r.index.swap(vec);
+    r.idx_pinned = pinned;
+    r.idx_off = ioff;
     r.n = n;
     r.out_bytes = bytes;
-    r.t_arrival = now_us();
+    r.t_arrival = t_arrival;
     srv->pending.push_back(id);
     lk.unlock();
     srv->cv_work.notify_all();
@@ -411,5 +443,20 @@ extern "C" int ibp_server_set_budget_override(ibp_server *srv, size_t bytes)
         srv->budget_override = bytes;
     }
     srv->cv_work.notify_all();
+    return IBP_OK;
+}
+
+extern "C" int ibp_server_request_times(ibp_server *srv, int64_t req_id, double *times)
+{
+    IBP_CHECK(srv && times, "null argument");
+    std::lock_guard<std::mutex> lk(srv->mu);
+    auto it = srv->reqs.find(req_id);
+    IBP_CHECK(it != srv->reqs.end(), "unknown request %lld", (long long)req_id);
+    const Req &r = it->second;
+    times[0] = r.t_arrival;
+    times[1] = r.t_admit;
+    times[2] = r.t_launch;
+    times[3] = r.t_issued;
+    times[4] = r.t_done;
     return IBP_OK;
 }

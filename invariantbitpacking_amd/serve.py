@@ -62,6 +62,12 @@ class Server:
         t = torch.as_tensor(_ArenaView(out.value, (n, v), typestr), device="cuda")
         return t, lat.value
 
+    def request_times(self, req):
+        """(arrival, admitted, launch start, launch issued, done) in host microseconds."""
+        t = (ctypes.c_double * 5)()
+        _capi.call("ibp_server_request_times", self._srv, req[0], t)
+        return tuple(t)
+
     def release(self, req):
         _capi.call("ibp_server_release", self._srv, req[0])
 

@@ -54,16 +54,21 @@ def gnn_minibatch_indices(num_rows, batch=200000, step=0):
     return torch.randint(0, num_rows, (batch,), generator=g, dtype=torch.int64)
 
 
-def dlrm_tables(num_tables=26, scale=1.0, seed=100):
+def dlrm_table_rows(num_tables=26, scale=1.0, seed=100):
+    """Row counts of the tables: log-uniform in [1e3, 1e7] (seeded), times scale."""
     rng = np.random.default_rng(seed)
     rows = np.round(10 ** rng.uniform(3, 7, size=num_tables)).astype(np.int64)
+    return [max(64, int(round(r * scale))) for r in rows]
+
+
+def dlrm_tables(num_tables=26, scale=1.0, seed=100, device="cpu"):
+    """fp32 [rows x 128] ~ N(0, 0.01) quantised to 2^-12, viewed as int64 [rows x 64]."""
     tables = []
-    for t, r in enumerate(rows):
-        r = max(64, int(round(r * scale)))
-        g = np.random.default_rng(seed + 1000 + t)
-        x = g.normal(0.0, 0.01, size=(r, 128)).astype(np.float32)
-        x = (np.round(x * 4096.0) / 4096.0).astype(np.float32)
-        tables.append(torch.from_numpy(x).view(torch.int64))
+    for t, r in enumerate(dlrm_table_rows(num_tables, scale, seed)):
+        g = torch.Generator(device=device).manual_seed(seed + 1000 + t)
+        x = torch.randn((r, 128), generator=g, device=device, dtype=torch.float32).mul_(0.01)
+        x = torch.round(x * 4096.0).div_(4096.0)
+        tables.append(x.view(torch.int64))
     return tables
 
 
