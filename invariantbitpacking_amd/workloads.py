@@ -85,16 +85,19 @@ def dlrm_indices(table_rows, batch=65536, seed=101):
     return out
 
 
-def llm_weight_blocks(layers=1, scale=1.0, seed=200):
-    """Per layer: 4 x [4096 x 4096] and 3 x [4096 x 11008] fp16 blocks as int64."""
+def llm_weight_blocks(layers=1, scale=1.0, seed=200, device="cuda"):
+    """Per layer: 4 x [4096 x 4096] and 3 x [4096 x 11008] fp16 N(0, 0.02)
+    blocks as int64 views, generated on `device` and returned on the CPU."""
+    if device == "cuda" and not torch.cuda.is_available():
+        device = "cpu"
     out = []
     for layer in range(layers):
-        g = torch.Generator().manual_seed(seed + layer)
+        g = torch.Generator(device=device).manual_seed(seed + layer)
         rows = max(4, int(4096 * scale))
         blocks = []
         for shape in [(rows, 4096)] * 4 + [(rows, 11008)] * 3:
-            w = (torch.randn(shape, generator=g, dtype=torch.float32) * 0.02).to(torch.float16)
-            blocks.append(w.view(torch.int64))
+            w = (torch.randn(shape, generator=g, device=device, dtype=torch.float32) * 0.02).to(torch.float16)
+            blocks.append(w.view(torch.int64).cpu())
         out.append(blocks)
     return out
 
