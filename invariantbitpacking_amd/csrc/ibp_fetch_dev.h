@@ -35,6 +35,14 @@ struct FetchParams {
     unsigned long long *err;
 };
 
+// Fetched rows are written once and not read back by the kernel: streaming
+// (non-temporal) stores keep them from displacing compressed rows in L2/MALL.
+template <typename T>
+__device__ __forceinline__ void store_row_word(T *p, T v)
+{
+    __builtin_nontemporal_store(v, p);
+}
+
 template <typename IdxT>
 __device__ __forceinline__ int64_t load_index(const void *index, int64_t i)
 {
@@ -211,7 +219,7 @@ __device__ __forceinline__ void fetch_small_range(const FetchParams &p, const Sm
                 const T v = stream_bits<T>(w0, w1, o);
                 val = flag[r] ? T(col.bitval | small_deposit<T>(col, v)) : v;
             }
-            if (i < end && lane < V) out[i * V + lane] = ok ? val : T(0);
+            if (i < end && lane < V) store_row_word<T>(&out[i * V + lane], ok ? val : T(0));
         }
         // The ballot needs the whole wave active, so it is taken before the lane-0 branch.
         const uint64_t livemask = ballot(live);
@@ -255,7 +263,7 @@ __device__ __forceinline__ void fetch_large_row_from(const FetchParams &p, int64
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int64_t k = k0 + u * kWave;
-                if (k < V) dst[k] = a[u];
+                if (k < V) store_row_word<T>(&dst[k], a[u]);
             }
         }
         return;
@@ -305,7 +313,7 @@ __device__ __forceinline__ void fetch_large_row_from(const FetchParams &p, int64
         const T a1 = q + 1 < kWave ? s1 : X;
         const T v = stream_bits<T>(a0, a1, o);
         const T val = f ? T(b | deposit_msb<T>(v, T(~m))) : v;
-        if (valid) dst[j] = val;
+        if (valid) store_row_word<T>(&dst[j], val);
         cursor += tot;
     }
 }
