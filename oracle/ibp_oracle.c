@@ -195,16 +195,30 @@ int64_t oracle_row_size(int w, const void *row, int64_t vec_size, const void *ma
     return comp < vec_bytes ? comp : vec_bytes;
 }
 
+/* Mask row of source row r: clusters[r] (NULL: row 0, which is what the
+ * reference applies to every row, src/compress.cu:221-232). */
+static inline int64_t mask_row(const int32_t *clusters, int64_t r) { return clusters ? clusters[r] : 0; }
+
+void oracle_get_compress_size_clustered(int w, const void *data, int64_t num_vecs, int64_t vec_size,
+                                        const void *masks, const void *bitvals, const int32_t *clusters,
+                                        const int64_t *index, int64_t n, int64_t *sizes, uint64_t *ctr)
+{
+    (void)num_vecs;
+    const int64_t vb = vec_size * w;
+    for (int64_t i = 0; i < n; ++i) {
+        int64_t r = index ? index[i] : i; /* ibp_misc_kernels.cuh:18-24 */
+        const int64_t c = mask_row(clusters, r);
+        sizes[i] = oracle_row_size(w, (const uint8_t *)data + r * vb, vec_size, (const uint8_t *)masks + c * vb,
+                                   (const uint8_t *)bitvals + c * vb);
+        if (ctr) *ctr += (uint64_t)sizes[i]; /* :27-30, never reset */
+    }
+}
+
 void oracle_get_compress_size(int w, const void *data, int64_t num_vecs, int64_t vec_size,
                               const void *mask, const void *bitval, const int64_t *index,
                               int64_t n, int64_t *sizes, uint64_t *ctr)
 {
-    (void)num_vecs;
-    for (int64_t i = 0; i < n; ++i) {
-        int64_t r = index ? index[i] : i; /* ibp_misc_kernels.cuh:18-24 */
-        sizes[i] = oracle_row_size(w, (const uint8_t *)data + r * vec_size * w, vec_size, mask, bitval);
-        if (ctr) *ctr += (uint64_t)sizes[i]; /* :27-30, never reset */
-    }
+    oracle_get_compress_size_clustered(w, data, num_vecs, vec_size, mask, bitval, NULL, index, n, sizes, ctr);
 }
 
 /* ---- slot codec ---------------------------------------------------------- */
@@ -315,9 +329,9 @@ void oracle_unpack_row(int w, const void *slot, int64_t vec_size, const void *ma
     }
 }
 
-void oracle_compress(int w, const void *input, void *output, int64_t num_vecs, int64_t vec_size,
-                     const void *mask, const void *bitval, const int64_t *index, int64_t n,
-                     int32_t *bitmask)
+void oracle_compress_clustered(int w, const void *input, void *output, int64_t num_vecs,
+                               int64_t vec_size, const void *masks, const void *bitvals,
+                               const int32_t *clusters, const int64_t *index, int64_t n, int32_t *bitmask)
 {
     (void)num_vecs;
     const int64_t vec_bytes = vec_size * w;
@@ -327,10 +341,12 @@ void oracle_compress(int w, const void *input, void *output, int64_t num_vecs, i
         int64_t r = index ? index[i] : i; /* ibp_compress_kernel.cuh:22-25 */
         const uint8_t *src = (const uint8_t *)input + r * vec_bytes;
         uint8_t *dst = (uint8_t *)output + i * vec_bytes;
+        const int64_t c = mask_row(clusters, r);
+        const uint8_t *m = (const uint8_t *)masks + c * vec_bytes, *b = (const uint8_t *)bitvals + c * vec_bytes;
         /* :28-31: pack only if strictly smaller (int32 closed form, same value). */
-        int64_t compressed = oracle_row_size(w, src, vec_size, mask, bitval);
+        int64_t compressed = oracle_row_size(w, src, vec_size, m, b);
         if (compressed < vec_bytes) {
-            oracle_pack_row(w, src, vec_size, mask, bitval, slot);
+            oracle_pack_row(w, src, vec_size, m, b, slot);
             memcpy(dst, slot, (size_t)vec_bytes); /* :36-39 */
             bitmask[i / 32] |= (int32_t)(1u << (i % 32)); /* :40-46 */
         } else if (dst != src) {
@@ -340,9 +356,16 @@ void oracle_compress(int w, const void *input, void *output, int64_t num_vecs, i
     free(slot);
 }
 
-void oracle_decompress_fetch(int w, const void *comp, int64_t num_vecs, int64_t vec_size,
-                             const void *mask, const void *bitval, const int32_t *bitmask,
-                             const int64_t *index, int64_t n, void *out)
+void oracle_compress(int w, const void *input, void *output, int64_t num_vecs, int64_t vec_size,
+                     const void *mask, const void *bitval, const int64_t *index, int64_t n,
+                     int32_t *bitmask)
+{
+    oracle_compress_clustered(w, input, output, num_vecs, vec_size, mask, bitval, NULL, index, n, bitmask);
+}
+
+void oracle_decompress_fetch_clustered(int w, const void *comp, int64_t num_vecs, int64_t vec_size,
+                                       const void *masks, const void *bitvals, const int32_t *clusters,
+                                       const int32_t *bitmask, const int64_t *index, int64_t n, void *out)
 {
     (void)num_vecs;
     const int64_t vec_bytes = vec_size * w;
@@ -350,12 +373,81 @@ void oracle_decompress_fetch(int w, const void *comp, int64_t num_vecs, int64_t 
         int64_t r = index ? index[k] : k;
         const uint8_t *src = (const uint8_t *)comp + r * vec_bytes;
         uint8_t *dst = (uint8_t *)out + k * vec_bytes;
+        const int64_t c = mask_row(clusters, r);
         /* ibp_decompress_kernel.cuh:95-102 */
         if ((uint32_t)bitmask[r / 32] & (1u << (r % 32)))
-            oracle_unpack_row(w, src, vec_size, mask, bitval, dst);
+            oracle_unpack_row(w, src, vec_size, (const uint8_t *)masks + c * vec_bytes,
+                              (const uint8_t *)bitvals + c * vec_bytes, dst);
         else
             memcpy(dst, src, (size_t)vec_bytes);
     }
+}
+
+void oracle_decompress_fetch(int w, const void *comp, int64_t num_vecs, int64_t vec_size,
+                             const void *mask, const void *bitval, const int32_t *bitmask,
+                             const int64_t *index, int64_t n, void *out)
+{
+    oracle_decompress_fetch_clustered(w, comp, num_vecs, vec_size, mask, bitval, NULL, bitmask, index, n, out);
+}
+
+/* ---- condensed stream -------------------------------------------------------- */
+
+int64_t oracle_compress_condensed(int w, const void *input, int64_t num_vecs, int64_t vec_size,
+                                  const void *masks, const void *bitvals, const int32_t *clusters,
+                                  const int64_t *index, int64_t n, void *out, int64_t *offsets)
+{
+    (void)num_vecs;
+    const int64_t vec_bytes = vec_size * w;
+    uint8_t *slot = (uint8_t *)malloc((size_t)vec_bytes);
+    int64_t start = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        int64_t r = index ? index[i] : i; /* ibp_compress_kernel.cuh:76-80 */
+        const uint8_t *src = (const uint8_t *)input + r * vec_bytes;
+        const int64_t c = mask_row(clusters, r);
+        const uint8_t *m = (const uint8_t *)masks + c * vec_bytes, *b = (const uint8_t *)bitvals + c * vec_bytes;
+        /* src/compress.cu:276-285: sizes, then their inclusive scan. */
+        const int64_t size = oracle_row_size(w, src, vec_size, m, b);
+        /* ibp_compress_kernel.cuh:66-73,82-92: row i occupies [offsets[i-1],
+         * offsets[i]); a row whose size is V*w is copied raw, any other is the
+         * first `size` bytes of its slot (the rest of the slot is zero). */
+        if (out) {
+            if (size != vec_bytes) {
+                oracle_pack_row(w, src, vec_size, m, b, slot);
+                memcpy((uint8_t *)out + start, slot, (size_t)size);
+            } else {
+                memcpy((uint8_t *)out + start, src, (size_t)vec_bytes);
+            }
+        }
+        start += size;
+        offsets[i] = start;
+    }
+    free(slot);
+    return start;
+}
+
+void oracle_decompress_condensed(int w, const void *comp, const int64_t *offsets, int64_t num_rows,
+                                 int64_t vec_size, const void *masks, const void *bitvals,
+                                 const int32_t *clusters, const int64_t *index, int64_t n, void *out)
+{
+    (void)num_rows;
+    const int64_t vec_bytes = vec_size * w;
+    uint8_t *slot = (uint8_t *)malloc((size_t)vec_bytes);
+    for (int64_t k = 0; k < n; ++k) {
+        const int64_t r = index ? index[k] : k;
+        const int64_t lo = r == 0 ? 0 : offsets[r - 1], len = offsets[r] - lo;
+        uint8_t *dst = (uint8_t *)out + k * vec_bytes;
+        const int64_t c = mask_row(clusters, r);
+        if (len == vec_bytes) {
+            memcpy(dst, (const uint8_t *)comp + lo, (size_t)vec_bytes);
+        } else {
+            /* the stored prefix of the slot; its zero tail was not stored */
+            memset(slot, 0, (size_t)vec_bytes);
+            memcpy(slot, (const uint8_t *)comp + lo, (size_t)len);
+            oracle_unpack_row(w, slot, vec_size, (const uint8_t *)masks + c * vec_bytes,
+                              (const uint8_t *)bitvals + c * vec_bytes, dst);
+        }
+    }
+    free(slot);
 }
 
 /* ---- k-means --------------------------------------------------------------- */

@@ -79,6 +79,39 @@ void oracle_decompress_fetch(int w, const void *comp, int64_t num_vecs, int64_t 
                              const void *mask, const void *bitval, const int32_t *bitmask,
                              const int64_t *index, int64_t n, void *out);
 
+/* Per-cluster forms of the three functions above.  masks/bitvals are [K x V]
+ * (the preprocess_kmeans output); clusters (NULL: row 0 for every row, which
+ * is what the reference applies, src/compress.cu:221-232 and
+ * src/decompress.cu:130-134) names the mask row of each row: of each SOURCE
+ * row (dataset order) for size and compress, of each STORED row for fetch. */
+void oracle_get_compress_size_clustered(int w, const void *data, int64_t num_vecs, int64_t vec_size,
+                                        const void *masks, const void *bitvals, const int32_t *clusters,
+                                        const int64_t *index, int64_t n, int64_t *sizes, uint64_t *ctr);
+void oracle_compress_clustered(int w, const void *input, void *output, int64_t num_vecs,
+                               int64_t vec_size, const void *masks, const void *bitvals,
+                               const int32_t *clusters, const int64_t *index, int64_t n, int32_t *bitmask);
+void oracle_decompress_fetch_clustered(int w, const void *comp, int64_t num_vecs, int64_t vec_size,
+                                       const void *masks, const void *bitvals, const int32_t *clusters,
+                                       const int32_t *bitmask, const int64_t *index, int64_t n, void *out);
+
+/* src/compress.cu:238-313 + include/compress/ibp_compress_kernel.cuh:59-94
+ * (compress_condensed): rows index[i] (or i) stored back to back, row i in
+ * bytes [offsets[i-1], offsets[i]) (offsets[-1] = 0) -- the inclusive scan of
+ * the row sizes.  A row of size V*w is stored raw, any other as the first
+ * `size` bytes of its slot.  out (NULL: offsets only) must hold the returned
+ * total.  clusters as above (per source row). */
+int64_t oracle_compress_condensed(int w, const void *input, int64_t num_vecs, int64_t vec_size,
+                                  const void *masks, const void *bitvals, const int32_t *clusters,
+                                  const int64_t *index, int64_t n, void *out, int64_t *offsets);
+
+/* Inverse of oracle_compress_condensed (the reference has none: its
+ * `decompress` binding is commented out, src/ibp_api.cu:134): out[k] = stored
+ * row index[k] (or k), unpacked when its stored length differs from V*w.
+ * clusters per stored row. */
+void oracle_decompress_condensed(int w, const void *comp, const int64_t *offsets, int64_t num_rows,
+                                 int64_t vec_size, const void *masks, const void *bitvals,
+                                 const int32_t *clusters, const int64_t *index, int64_t n, void *out);
+
 /* libstdc++ std::random_shuffle(first, last) over 0..n-1 driven by rand()
  * (include/preproc/ibp_preproc_host.cuh:176-180). */
 void oracle_random_shuffle(int32_t *indices, int64_t n);

@@ -49,6 +49,14 @@ def lib():
         L.oracle_unpack_row.argtypes = [ctypes.c_int, _P, _I64, _P, _P, _P]
         L.oracle_compress.argtypes = [ctypes.c_int, _P, _P, _I64, _I64, _P, _P, _P, _I64, _P]
         L.oracle_decompress_fetch.argtypes = [ctypes.c_int, _P, _I64, _I64, _P, _P, _P, _P, _I64, _P]
+        L.oracle_get_compress_size_clustered.argtypes = [ctypes.c_int, _P, _I64, _I64, _P, _P, _P, _P, _I64,
+                                                         _P, ctypes.POINTER(ctypes.c_uint64)]
+        L.oracle_compress_clustered.argtypes = [ctypes.c_int, _P, _P, _I64, _I64, _P, _P, _P, _P, _I64, _P]
+        L.oracle_decompress_fetch_clustered.argtypes = [ctypes.c_int, _P, _I64, _I64, _P, _P, _P, _P, _P, _I64,
+                                                        _P]
+        L.oracle_compress_condensed.argtypes = [ctypes.c_int, _P, _I64, _I64, _P, _P, _P, _P, _I64, _P, _P]
+        L.oracle_compress_condensed.restype = ctypes.c_int64
+        L.oracle_decompress_condensed.argtypes = [ctypes.c_int, _P, _P, _I64, _I64, _P, _P, _P, _P, _I64, _P]
         L.oracle_random_shuffle.argtypes = [_P, _I64]
         L.oracle_kmeans.argtypes = [ctypes.c_int, _P, _I64, _I64, ctypes.c_int, _I64, _P, _P, _P]
         L.oracle_sweep_last.argtypes = [ctypes.POINTER(ctypes.c_int)]
@@ -114,16 +122,23 @@ def row_size(row, mask, bitval):
                                      _ptr(_c(mask)), _ptr(_c(bitval))))
 
 
-def get_compress_size(data, mask, bitval, index=None, ctr=None):
-    """Returns (sizes int64[n], ctr) with ctr the counter after the addition."""
+def _clusters(clusters):
+    return None if clusters is None else _c(np.asarray(clusters, dtype=np.int32))
+
+
+def get_compress_size(data, mask, bitval, index=None, ctr=None, clusters=None):
+    """Returns (sizes int64[n], ctr) with ctr the counter after the addition.
+    clusters: mask row of each source row (mask/bitval [K x V])."""
     data = _c(data)
     n_rows, v = data.shape
     idx = None if index is None else _c(np.asarray(index, dtype=np.int64))
     n = n_rows if idx is None else idx.shape[0]
     sizes = np.zeros(n, dtype=np.int64)
     c = None if ctr is None else ctypes.c_uint64(int(ctr))
-    lib().oracle_get_compress_size(data.itemsize, _ptr(data), n_rows, v, _ptr(_c(mask)), _ptr(_c(bitval)),
-                                   _ptr(idx), n, _ptr(sizes), None if c is None else ctypes.byref(c))
+    cl = _clusters(clusters)
+    lib().oracle_get_compress_size_clustered(data.itemsize, _ptr(data), n_rows, v, _ptr(_c(mask)),
+                                             _ptr(_c(bitval)), _ptr(cl), _ptr(idx), n, _ptr(sizes),
+                                             None if c is None else ctypes.byref(c))
     return sizes, (None if c is None else c.value)
 
 
@@ -141,7 +156,7 @@ def unpack_row(slot, mask, bitval):
     return out
 
 
-def compress(data, mask, bitval, index=None):
+def compress(data, mask, bitval, index=None, clusters=None):
     """Out-of-place compress_inplace: returns (slots [n x V], bitmask int32)."""
     data = _c(data)
     n_rows, v = data.shape
@@ -149,8 +164,8 @@ def compress(data, mask, bitval, index=None):
     n = n_rows if idx is None else idx.shape[0]
     out = np.zeros((n, v), dtype=data.dtype)
     bitmask = np.zeros((n + 31) // 32, dtype=np.int32)
-    lib().oracle_compress(data.itemsize, _ptr(data), _ptr(out), n_rows, v, _ptr(_c(mask)), _ptr(_c(bitval)),
-                          _ptr(idx), n, _ptr(bitmask))
+    lib().oracle_compress_clustered(data.itemsize, _ptr(data), _ptr(out), n_rows, v, _ptr(_c(mask)),
+                                    _ptr(_c(bitval)), _ptr(_clusters(clusters)), _ptr(idx), n, _ptr(bitmask))
     return out, bitmask
 
 
@@ -164,14 +179,45 @@ def compress_inplace(data, mask, bitval):
     return bitmask
 
 
-def decompress_fetch(comp, mask, bitval, bitmask, index=None):
+def decompress_fetch(comp, mask, bitval, bitmask, index=None, clusters=None):
+    """clusters: mask row of each stored row (mask/bitval [K x V])."""
     comp = _c(comp)
     n_rows, v = comp.shape
     idx = None if index is None else _c(np.asarray(index, dtype=np.int64))
     n = n_rows if idx is None else idx.shape[0]
     out = np.zeros((n, v), dtype=comp.dtype)
-    lib().oracle_decompress_fetch(comp.itemsize, _ptr(comp), n_rows, v, _ptr(_c(mask)), _ptr(_c(bitval)),
-                                  _ptr(_c(np.asarray(bitmask, dtype=np.int32))), _ptr(idx), n, _ptr(out))
+    lib().oracle_decompress_fetch_clustered(comp.itemsize, _ptr(comp), n_rows, v, _ptr(_c(mask)),
+                                            _ptr(_c(bitval)), _ptr(_clusters(clusters)),
+                                            _ptr(_c(np.asarray(bitmask, dtype=np.int32))), _ptr(idx), n, _ptr(out))
+    return out
+
+
+def compress_condensed(data, mask, bitval, index=None, clusters=None):
+    """Returns (stream, offsets): stream of dtype data.dtype holding the rows
+    back to back, offsets int64[n] the inclusive end byte offset of each row."""
+    data = _c(data)
+    n_rows, v = data.shape
+    w = data.itemsize
+    idx = None if index is None else _c(np.asarray(index, dtype=np.int64))
+    n = n_rows if idx is None else idx.shape[0]
+    offsets = np.zeros(n, dtype=np.int64)
+    cl = _clusters(clusters)
+    args = (_ptr(_c(mask)), _ptr(_c(bitval)), _ptr(cl), _ptr(idx), n)
+    total = lib().oracle_compress_condensed(w, _ptr(data), n_rows, v, *args, None, _ptr(offsets))
+    out = np.zeros(total // w, dtype=data.dtype)
+    lib().oracle_compress_condensed(w, _ptr(data), n_rows, v, *args, _ptr(out), _ptr(offsets))
+    return out, offsets
+
+
+def decompress_condensed(stream, offsets, vec_size, mask, bitval, index=None, clusters=None):
+    stream = _c(stream)
+    offsets = _c(np.asarray(offsets, dtype=np.int64))
+    idx = None if index is None else _c(np.asarray(index, dtype=np.int64))
+    n = offsets.shape[0] if idx is None else idx.shape[0]
+    out = np.zeros((n, vec_size), dtype=stream.dtype)
+    lib().oracle_decompress_condensed(stream.itemsize, _ptr(stream), _ptr(offsets), offsets.shape[0], vec_size,
+                                      _ptr(_c(mask)), _ptr(_c(bitval)), _ptr(_clusters(clusters)), _ptr(idx), n,
+                                      _ptr(out))
     return out
 
 
