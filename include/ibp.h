@@ -6,7 +6,9 @@
  * Work is enqueued on the caller's HIP stream (`stream` is a hipStream_t; NULL
  * is the null stream).  Unless stated otherwise a call does not synchronise.
  *
- * Element width `elem_bytes` is 1, 2, 4 or 8 (the slot codec takes 4 or 8).
+ * Element width `elem_bytes` is 1, 2, 4 or 8 bytes.  The reference's codec
+ * takes 4 or 8 (src/compress.cu:191-192); 1- and 2-byte elements use the
+ * same slot format with 8- and 16-bit words.
  * Index arrays hold `index_bytes` = 4 (int32) or 8 (int64) signed entries.
  * Data pointers may be device memory or pinned (hipHostMalloc) host memory.
  *
@@ -77,6 +79,29 @@ int ibp_preprocess_kmeans(int elem_bytes, const void *data, int64_t num_vecs, in
                           int num_clusters, float threshold, int64_t seed, void *masks,
                           void *bitvals, int32_t *clusters, void *stream);
 
+/* ---- row layouts -------------------------------------------------------------- */
+
+/* Optional description of how compressed rows are stored and coded, taken by
+ * the *_ex entry points (NULL = the reference's slot layout with mask row 0).
+ *  - offsets != NULL: CONDENSED stream (compress_condensed, src/compress.cu:
+ *    238-313): stored row s is bytes [offsets[s-1], offsets[s]) of the stream
+ *    (offsets[-1] = 0), i.e. offsets is the inclusive scan of the row sizes.
+ *    A row whose length is V*w is raw; any other is its slot cut at its size.
+ *    stream_bytes bounds every read of the stream.  No bitmask is needed.
+ *  - clusters != NULL: mask/bitval are [num_clusters x V] (preprocess_kmeans
+ *    output) and row r is coded with row clusters[r].  For size and compress
+ *    clusters is indexed by SOURCE row (the dataset's row, as preprocess_kmeans
+ *    returns it); for fetch by STORED row.  The reference applies mask row 0 to
+ *    every row (src/compress.cu:221-232, src/decompress.cu:130-134).
+ * Rows whose byte range or cluster id is invalid are fetched as zero rows and
+ * counted like out-of-range indices (ibp_index_errors). */
+typedef struct ibp_layout {
+    const int64_t *offsets;
+    int64_t stream_bytes;
+    const int32_t *clusters;
+    int64_t num_clusters;
+} ibp_layout;
+
 /* ---- compressed size and pack ----------------------------------------------- */
 
 /* Replaces get_compress_size (src/compress.cu:25-174) ->
@@ -87,6 +112,21 @@ int ibp_get_compress_size(int elem_bytes, const void *data, int64_t num_vecs, in
                           const void *mask, const void *bitval, const void *index,
                           int index_bytes, int64_t n, int64_t *sizes, uint64_t *total,
                           void *stream);
+/* ibp_get_compress_size with per-row mask rows (layout->clusters; offsets
+ * must be NULL).  layout NULL is ibp_get_compress_size. */
+int ibp_get_compress_size_ex(int elem_bytes, const void *data, int64_t num_vecs, int64_t vec_size,
+                             const void *mask, const void *bitval, const void *index,
+                             int index_bytes, int64_t n, int64_t *sizes, uint64_t *total,
+                             const ibp_layout *layout, void *stream);
+
+/* First half of compress_condensed (src/compress.cu:238-287): the row sizes
+ * of get_compress_size and their inclusive scan, into offsets [n] (device or
+ * pinned host).  offsets[n-1] is the stream's byte length; the caller reads it
+ * (after synchronising) to size the stream.  layout: clusters only.
+ * Asynchronous. */
+int ibp_condensed_offsets(int elem_bytes, const void *data, int64_t num_vecs, int64_t vec_size,
+                          const void *mask, const void *bitval, const void *index, int index_bytes,
+                          int64_t n, int64_t *offsets, const ibp_layout *layout, void *stream);
 
 /* Replaces compress_inplace (src/compress.cu:176-236) -> ibp::compress_inplace
  * (include/compress/ibp_compress_host.cuh:8-27).  Slot i of `output` receives
@@ -97,6 +137,17 @@ int ibp_get_compress_size(int elem_bytes, const void *data, int64_t num_vecs, in
 int ibp_compress(int elem_bytes, const void *input, void *output, int64_t num_vecs,
                  int64_t vec_size, const void *mask, const void *bitval, const void *index,
                  int index_bytes, int64_t n, int32_t *bitmask, void *stream);
+/* ibp_compress with a layout.  layout->offsets (from ibp_condensed_offsets)
+ * makes it the second half of compress_condensed (src/compress.cu:288-312 ->
+ * compress_condensed_kernel, include/compress/ibp_compress_kernel.cuh:59-94):
+ * row index[i] (or i) is written to bytes [offsets[i-1], offsets[i]) of
+ * `output` (the stream, layout->stream_bytes long, must not overlap input);
+ * bitmask may then be NULL, otherwise it is written as for slots.
+ * layout->clusters selects each source row's mask row. */
+int ibp_compress_ex(int elem_bytes, const void *input, void *output, int64_t num_vecs,
+                    int64_t vec_size, const void *mask, const void *bitval, const void *index,
+                    int index_bytes, int64_t n, int32_t *bitmask, const ibp_layout *layout,
+                    void *stream);
 
 /* ---- fetch (decompress) ------------------------------------------------------- */
 
@@ -116,6 +167,16 @@ int ibp_decompress_fetch(int elem_bytes, void *out, const void *comp, int64_t nu
                          const int32_t *bitmask, const void *index, int index_bytes, int64_t n,
                          int comp_len, int nblks, int nthreads, int impl, int comp_loc,
                          void *stream);
+/* ibp_decompress_fetch with a layout.  With layout->offsets, comp is a
+ * condensed stream of num_vecs stored rows and bitmask may be NULL: this is
+ * the stream's decompress, which the reference declares but never built
+ * (src/ibp_api.cu:134).  With layout->clusters, stored row s is unpacked with
+ * mask row clusters[s].  impl 2/3/4 take neither. */
+int ibp_decompress_fetch_ex(int elem_bytes, void *out, const void *comp, int64_t num_vecs,
+                            int64_t vec_size, const void *mask, const void *bitval,
+                            const int32_t *bitmask, const void *index, int index_bytes, int64_t n,
+                            int comp_len, int nblks, int nthreads, int impl, int comp_loc,
+                            const ibp_layout *layout, void *stream);
 
 /* Launch shape of ibp_decompress_fetch for rows of <= 64 elements: rows kept
  * in flight per wave (2, 4, 8 or 16) and resident blocks per CU (1..32).
@@ -207,6 +268,15 @@ int ibp_server_stats_get(ibp_server *srv, ibp_server_stats *st);
 int ibp_server_request_times(ibp_server *srv, int64_t req_id, double *times);
 /* Test hook: pretend the arena has only `bytes` free (0 disables). */
 int ibp_server_set_budget_override(ibp_server *srv, size_t bytes);
+
+/* ---- file loading ------------------------------------------------------------ */
+
+/* The copy step of read_shared (src/ibp_api.cu:52-122): reads the whole file
+ * at `path` into dst (host memory, `capacity` bytes; bytes past the file's end
+ * are left as they are) with `threads` reader threads (<= 0: 16).  A file larger
+ * than capacity is IBP_ERR_INVALID (the reference copies past the tensor).
+ * *bytes_read receives the file size.  Synchronous, no GPU work. */
+int ibp_read_file(const char *path, void *dst, size_t capacity, int threads, size_t *bytes_read);
 
 /* ---- self-test of the element-level bit arithmetic on the host ------------ */
 /* Runs the codec's element functions (compiled for host) against a brute-force

@@ -10,6 +10,12 @@ memory, pinned host memory and the current HIP stream only.
     mask, bitval = ibp.preprocess(data, 0.9)
     bitmask = ibp.compress_inplace(pinned, mask, bitval)
     rows = ibp.decompress_fetch(pinned, mask, bitval, bitmask, torch.device("cuda"), index_arr=idx)
+
+Beyond the reference (keyword-only, so positional calls are unchanged):
+clusters= on the size, compress and fetch calls codes each row with its own
+preprocess_kmeans mask row; compress_condensed / compress and decompress
+give and read the condensed stream; 1- and 2-byte elements are accepted by
+every call.
 """
 import ctypes
 
@@ -19,7 +25,8 @@ from . import _capi
 from ._capi import IBPError  # noqa: F401  (re-exported)
 
 __all__ = ["print_debug", "preprocess", "preprocess_kmeans", "get_compress_size", "get_single_compress_len",
-           "compress_inplace", "compress", "decompress_fetch", "IBPError", "version", "index_errors"]
+           "compress_inplace", "compress_condensed", "compress", "decompress_fetch", "decompress", "read_shared",
+           "IBPError", "version", "index_errors"]
 
 _SIGNED = {1: torch.int8, 2: torch.int16, 4: torch.int32, 8: torch.int64}
 
@@ -63,6 +70,30 @@ def _index_arg(index_arr, require_cuda):
         idx = idx.to(torch.int64)
     idx = idx.contiguous()
     return idx, idx.element_size()
+
+
+def _clusters_arg(clusters, mask, rows):
+    """(int32 tensor kept alive, number of mask rows) for a clusters argument."""
+    if clusters is None:
+        return None, 1
+    _check(_accessible(clusters), "Clusters tensor must accessible by CUDA device (GPU or pinned memory)")
+    _check(clusters.dim() == 1 and clusters.numel() >= rows, "Clusters must be 1D with one entry per row")
+    _check(mask.dim() == 2, "With clusters, mask and bitval must be 2D [num_clusters x vec_size]")
+    cl = clusters if clusters.dtype == torch.int32 else clusters.to(torch.int32)
+    return cl.contiguous(), mask.size(0)
+
+
+def _layout(offsets=None, stream_bytes=0, clusters=None, num_clusters=1):
+    if offsets is None and clusters is None:
+        return None
+    return ctypes.byref(_capi.Layout(None if offsets is None else offsets.data_ptr(), int(stream_bytes),
+                                     None if clusters is None else clusters.data_ptr(), int(num_clusters)))
+
+
+def _host_temps(*pairs):
+    """True if some argument was converted into a temporary in host memory:
+    such a temporary must outlive the asynchronous kernels that read it."""
+    return any(a is not None and b is not None and a is not b and not b.is_cuda for a, b in pairs)
 
 
 def version():
@@ -127,26 +158,31 @@ def preprocess_kmeans(dataset, centroids, threshold=None, *, seed=None):
     return masks, bitvals, clusters
 
 
-def _mask_check(dataset, mask, bitval):
+def _mask_check(dataset, mask, bitval, v=None):
     _check(_accessible(mask), "Mask tensor must accessible by CUDA device (GPU or pinned memory)")
     _check(_accessible(bitval), "Bitval tensor must accessible by CUDA device (GPU or pinned memory)")
     w = dataset.element_size()
     _check(mask.element_size() == w and bitval.element_size() == w,
-           "Mask and bitval tensors must have the same datatype size (4-byte or 8-byte) as input dataset")
+           "Mask and bitval tensors must have the same datatype size as input dataset")
     _check(mask.is_contiguous() and bitval.is_contiguous(), "Mask and bitval tensors must be contiguous")
-    v = dataset.size(0) if dataset.dim() == 1 else dataset.size(1)
+    if v is None:
+        v = dataset.size(0) if dataset.dim() == 1 else dataset.size(1)
     _check(mask.numel() >= v and bitval.numel() >= v, "Mask and bitval must hold at least vec_size elements")
+    _check(mask.dim() == 1 or mask.size(-1) == v, "Mask rows must have vec_size elements")
+    _check(mask.shape == bitval.shape, "Mask and bitval must have the same shape")
 
 
 def _shape(dataset):
     return (1, dataset.size(0)) if dataset.dim() == 1 else (dataset.size(0), dataset.size(1))
 
 
-def get_compress_size(dataset, mask, bitval, index_arr=None, compress_total=None):
+def get_compress_size(dataset, mask, bitval, index_arr=None, compress_total=None, *, clusters=None):
     """Compressed bytes of each row (int64 [n]; src/compress.cu:25-174).
 
     compress_total (1-element int64 tensor) is incremented by the total and
-    never reset.  Asynchronous unless compress_total lives in host memory.
+    never reset.  clusters (extension): mask row of each dataset row, mask and
+    bitval [K, V].  Asynchronous unless compress_total lives in host memory or
+    an argument had to be converted in host memory.
     """
     _check(_accessible(dataset), "Input tensor must accessible by CUDA device (GPU or pinned memory)")
     _check(dataset.dim() in (1, 2), "Input tensor must be 1D [vec_size] or 2D [num_vecs x vec_size]")
@@ -165,14 +201,17 @@ def get_compress_size(dataset, mask, bitval, index_arr=None, compress_total=None
             total_dev = compress_total
         else:
             total_dev = torch.zeros(1, dtype=torch.int64, device="cuda")
+    cl, k = _clusters_arg(clusters, mask, num_vecs)
     if dataset.is_cuda:
         sizes = torch.empty(n, dtype=torch.int64, device=dataset.device)
     else:
         sizes = torch.empty(n, dtype=torch.int64, pin_memory=True)
-    _capi.call("ibp_get_compress_size", w, _ptr(dataset), num_vecs, v, _ptr(mask), _ptr(bitval), _ptr(idx), ib, n,
-               _ptr(sizes), _ptr(total_dev), _stream())
+    _capi.call("ibp_get_compress_size_ex", w, _ptr(dataset), num_vecs, v, _ptr(mask), _ptr(bitval), _ptr(idx), ib,
+               n, _ptr(sizes), _ptr(total_dev), _layout(clusters=cl, num_clusters=k), _stream())
     if compress_total is not None and total_dev is not compress_total:
         compress_total += total_dev.to(compress_total.device).to(compress_total.dtype)
+    if _host_temps((index_arr, idx), (clusters, cl)):
+        torch.cuda.current_stream().synchronize()
     return sizes
 
 
@@ -186,19 +225,22 @@ def get_single_compress_len(dataset, mask, bitval, index_arr=None):
     return int(comp_size / og_size * dataset.size(1))
 
 
-def compress_inplace(dataset, mask, bitval, index_arr=None):
+def compress_inplace(dataset, mask, bitval, index_arr=None, *, clusters=None):
     """Packs rows in place; returns the int32 bitmask [ceil(n/32)] of packed rows
     (src/compress.cu:176-236).  With index_arr, slot i receives row index_arr[i]
     (rows are gathered first, so the result does not depend on execution order).
+    clusters (extension): mask row of each dataset row, mask and bitval [K, V].
+    Synchronises the current stream.
     """
     _check(_accessible(dataset), "Input tensor must accessible by CUDA device (GPU or pinned memory)")
     _check(dataset.dim() in (1, 2), "Input tensor must be 1D [vec_size] or 2D [num_vecs x vec_size]")
     _check(dataset.is_contiguous(), "Input tensor must be contiguous")
     w = dataset.element_size()
-    _check(w in (4, 8), "Input tensor must be 4-byte or 8-byte datatype")
+    _check(w in (1, 2, 4, 8), "Input tensor must be 1, 2, 4 or 8-byte datatype")
     _mask_check(dataset, mask, bitval)
     num_vecs, v = _shape(dataset)
     idx, _ = _index_arg(index_arr, require_cuda=False)
+    cl, k = _clusters_arg(clusters, mask, num_vecs)
     n = num_vecs if idx is None else idx.size(0)
     bitmask = torch.empty((n + 31) // 32, dtype=torch.int32, device="cuda")
     src = dataset
@@ -208,35 +250,81 @@ def compress_inplace(dataset, mask, bitval, index_arr=None):
         src = rows.index_select(0, idx.to(rows.device).long()).contiguous()
         if not src.is_cuda:
             src = src.pin_memory()
-    _capi.call("ibp_compress", w, _ptr(src), _ptr(dataset), n, v, _ptr(mask), _ptr(bitval), None, 0, n,
-               _ptr(bitmask), _stream())
-    if src is not dataset:
-        torch.cuda.current_stream().synchronize()  # keep the gathered rows alive until packed
+        if cl is not None:  # the gathered rows keep their own mask rows
+            cl = cl.index_select(0, idx.to(cl.device).long()).contiguous()
+            if not cl.is_cuda:
+                cl = cl.pin_memory()
+    _capi.call("ibp_compress_ex", w, _ptr(src), _ptr(dataset), n, v, _ptr(mask), _ptr(bitval), None, 0, n,
+               _ptr(bitmask), _layout(clusters=cl, num_clusters=k), _stream())
+    torch.cuda.current_stream().synchronize()  # in place, like the reference; keeps temporaries alive
     return bitmask
 
 
-def compress(dataset, mask, bitval, index_arr=None):
-    """The reference's `ibp.compress` calls a binding that does not exist
-    (ibp/__init__.py:35-36 vs src/ibp_api.cu:131)."""
-    raise NotImplementedError("ibp.compress (condensed stream) is not part of this build")
+def compress_condensed(dataset, mask, bitval, index_arr=None, *, clusters=None):
+    """Condensed stream of the rows and their byte offsets
+    (src/compress.cu:238-313; binding src/ibp_api.cu:131).
+
+    Returns (stream, byte_offsets): stream is a 1D tensor of the dataset's
+    dtype on the dataset's device (pinned if the dataset is on the host) holding
+    row index_arr[i] (or i) in bytes [byte_offsets[i-1], byte_offsets[i]);
+    byte_offsets (int64 [n], same placement) is the inclusive scan of the row
+    sizes.  A row of size V*w is stored raw, any other as its slot cut at its
+    size.  clusters (extension): mask row of each dataset row.  Synchronises
+    (the stream's length is read from the offsets, as in the reference).
+    """
+    _check(_accessible(dataset), "Input tensor must accessible by CUDA device (GPU or pinned memory)")
+    _check(dataset.dim() in (1, 2), "Input tensor must be 1D [vec_size] or 2D [num_vecs x vec_size]")
+    _check(dataset.is_contiguous(), "Input tensor must be contiguous")
+    w = dataset.element_size()
+    _check(w in (1, 2, 4, 8), "Input tensor must be 1, 2, 4 or 8-byte datatype")
+    _mask_check(dataset, mask, bitval)
+    num_vecs, v = _shape(dataset)
+    idx, ib = _index_arg(index_arr, require_cuda=False)
+    cl, k = _clusters_arg(clusters, mask, num_vecs)
+    n = num_vecs if idx is None else idx.size(0)
+    on_dev = dataset.is_cuda
+    offsets = (torch.empty(n, dtype=torch.int64, device=dataset.device) if on_dev
+               else torch.empty(n, dtype=torch.int64, pin_memory=True))
+    lay = _layout(clusters=cl, num_clusters=k)
+    _capi.call("ibp_condensed_offsets", w, _ptr(dataset), num_vecs, v, _ptr(mask), _ptr(bitval), _ptr(idx), ib, n,
+               _ptr(offsets), lay, _stream())
+    torch.cuda.current_stream().synchronize()
+    total = int(offsets[-1]) if n > 0 else 0
+    _check(total % w == 0, "internal error: stream length %d is not a multiple of %d" % (total, w))
+    stream = (torch.empty(total // w, dtype=dataset.dtype, device=dataset.device) if on_dev
+              else torch.empty(total // w, dtype=dataset.dtype, pin_memory=True))
+    if n > 0:
+        _capi.call("ibp_compress_ex", w, _ptr(dataset), _ptr(stream), num_vecs, v, _ptr(mask), _ptr(bitval),
+                   _ptr(idx), ib, n, None, _layout(offsets, total, cl, k), _stream())
+    if _host_temps((index_arr, idx), (clusters, cl)):
+        torch.cuda.current_stream().synchronize()
+    return stream, offsets
+
+
+def compress(dataset, mask, bitval, index_arr=None, *, clusters=None):
+    """`ibp.compress` (ibp/__init__.py:35-36) calls `ibp_cuda.compress`, which the
+    reference never binds; the binding it means is compress_condensed
+    (src/ibp_api.cu:131), which this calls."""
+    return compress_condensed(dataset, mask, bitval, index_arr, clusters=clusters)
 
 
 def decompress_fetch(comp_dataset, mask, bitval, bitmask, device, comp_len=None, index_arr=None, nblks=None,
-                     nthds=None, imp=None, output_tensor=None):
+                     nthds=None, imp=None, output_tensor=None, *, clusters=None):
     """Rows of the compressed dataset, unpacked (src/decompress.cu:10-156).
 
     out[k] = row index_arr[k] (all rows in order without an index).  The
     compressed data may be pinned host memory (read over PCIe inside the
     kernel) or GPU memory.  comp_len is a hint (expected compressed row length
     in elements); imp selects the reference's variants (-1/0/1/5 unpack, 2 raw
-    gather, 3/4 transfer-only baselines).  Asynchronous on the current stream.
+    gather, 3/4 transfer-only baselines).  clusters (extension): mask row of
+    each stored row, mask and bitval [K, V].  Asynchronous on the current stream.
     """
     _check(_accessible(comp_dataset), "Input tensor must accessible by CUDA device (GPU or pinned memory)")
     _check(comp_dataset.dim() in (1, 2),
            "Input tensor must be 1D [vec_size] or 2D [num_vecs x vec_size] got %d" % comp_dataset.dim())
     _check(comp_dataset.is_contiguous(), "Input tensor must be contiguous")
     w = comp_dataset.element_size()
-    _check(w in (4, 8), "Input tensor must be 4-byte or 8-byte datatype")
+    _check(w in (1, 2, 4, 8), "Input tensor must be 1, 2, 4 or 8-byte datatype")
     _mask_check(comp_dataset, mask, bitval)
     _check(bitmask.is_cuda or _accessible(bitmask), "Bitmask tensor must accessible by CUDA device")
     _check(bitmask.element_size() == 4 and bitmask.is_contiguous(), "Bitmask must be a contiguous int32 tensor")
@@ -244,17 +332,32 @@ def decompress_fetch(comp_dataset, mask, bitval, bitmask, device, comp_len=None,
     _check(bitmask.numel() >= (num_vecs + 31) // 32, "Bitmask too short for the dataset")
     idx, ib = _index_arg(index_arr, require_cuda=True)
     n = num_vecs if idx is None else idx.size(0)
+    clu, k = _clusters_arg(clusters, mask, num_vecs)
+    out = _fetch_output(comp_dataset.dtype, n, v, device, output_tensor)
+    _capi.call("ibp_decompress_fetch_ex", w, _ptr(out), _ptr(comp_dataset), num_vecs, v, _ptr(mask), _ptr(bitval),
+               _ptr(bitmask), _ptr(idx), ib, n, _hint(comp_len), 0 if nblks is None else int(nblks),
+               0 if nthds is None else int(nthds), -1 if imp is None else int(imp), _loc(comp_dataset),
+               _layout(clusters=clu, num_clusters=k), _stream())
+    if _host_temps((clusters, clu)):
+        torch.cuda.current_stream().synchronize()
+    return out
+
+
+def _hint(comp_len):
+    if comp_len is None:
+        return 0
+    cl = int(comp_len)
+    return cl if cl > 0 else 0
+
+
+def _fetch_output(dtype, n, v, device, output_tensor):
     if not isinstance(device, torch.device):
         device = torch.device(device)
-    cl = 0
-    if comp_len is not None:
-        cl = int(comp_len)
-        cl = cl if cl > 0 else 0
     if output_tensor is not None:
         _check(output_tensor.device == device or (device.type == "cuda" and output_tensor.is_cuda and
                                                   device.index is None),
                "Output tensor device does not match specified output device")
-        _check(output_tensor.dtype == comp_dataset.dtype, "Output tensor dtype does not match input dataset dtype")
+        _check(output_tensor.dtype == dtype, "Output tensor dtype does not match input dataset dtype")
         _check(output_tensor.dim() == 2 and output_tensor.size(0) == n and output_tensor.size(1) == v,
                "Output tensor shape does not match expected decompressed shape")
         _check(output_tensor.is_contiguous(), "Output tensor must be contiguous")
@@ -262,13 +365,59 @@ def decompress_fetch(comp_dataset, mask, bitval, bitmask, device, comp_len=None,
         if not out.is_cuda:
             _check(out.is_pinned(), "Output tensor on the host must be pinned")
     elif device.type == "cuda":
-        out = torch.empty((n, v), dtype=comp_dataset.dtype, device=device)
+        out = torch.empty((n, v), dtype=dtype, device=device)
     else:
-        out = torch.empty((n, v), dtype=comp_dataset.dtype, pin_memory=True)
-    _capi.call("ibp_decompress_fetch", w, _ptr(out), _ptr(comp_dataset), num_vecs, v, _ptr(mask), _ptr(bitval),
-               _ptr(bitmask), _ptr(idx), ib, n, cl, 0 if nblks is None else int(nblks),
-               0 if nthds is None else int(nthds), -1 if imp is None else int(imp), _loc(comp_dataset), _stream())
+        out = torch.empty((n, v), dtype=dtype, pin_memory=True)
     return out
+
+
+def decompress(comp_dataset, byte_offsets, mask, bitval, device, vec_size, index_arr=None, output_tensor=None,
+               comp_len=None, *, clusters=None):
+    """Rows of a condensed stream, unpacked: the inverse of compress_condensed.
+
+    The reference declares this binding but never built it (src/ibp_api.cu:134).
+    comp_dataset: the 1D stream (GPU or pinned); byte_offsets: its int64
+    offsets (GPU or pinned); vec_size: elements per row.  out[k] = stored row
+    index_arr[k] (all rows in order without an index).  clusters: mask row of
+    each stored row.  Asynchronous on the current stream.
+    """
+    _check(_accessible(comp_dataset), "Input tensor must accessible by CUDA device (GPU or pinned memory)")
+    _check(comp_dataset.dim() == 1 and comp_dataset.is_contiguous(), "The condensed stream must be a contiguous 1D tensor")
+    _check(_accessible(byte_offsets), "Offsets must accessible by CUDA device (GPU or pinned memory)")
+    _check(byte_offsets.dtype == torch.int64 and byte_offsets.dim() == 1 and byte_offsets.is_contiguous(),
+           "Offsets must be a contiguous 1D int64 tensor")
+    w = comp_dataset.element_size()
+    _check(w in (1, 2, 4, 8), "Input tensor must be 1, 2, 4 or 8-byte datatype")
+    v = int(vec_size)
+    _check(v > 0, "vec_size must be positive")
+    _mask_check(comp_dataset, mask, bitval, v)
+    num_rows = byte_offsets.numel()
+    idx, ib = _index_arg(index_arr, require_cuda=True)
+    n = num_rows if idx is None else idx.size(0)
+    clu, k = _clusters_arg(clusters, mask, num_rows)
+    out = _fetch_output(comp_dataset.dtype, n, v, device, output_tensor)
+    stream_bytes = comp_dataset.numel() * w
+    _capi.call("ibp_decompress_fetch_ex", w, _ptr(out), _ptr(comp_dataset), num_rows, v, _ptr(mask), _ptr(bitval),
+               None, _ptr(idx), ib, n, _hint(comp_len), 0, 0, -1, _loc(comp_dataset),
+               _layout(byte_offsets, stream_bytes, clu, k), _stream())
+    if _host_temps((clusters, clu)):
+        torch.cuda.current_stream().synchronize()
+    return out
+
+
+def read_shared(filename, shape, dtype, *, threads=0):
+    """A tensor of `shape` and `dtype` in process-shared memory holding the
+    contents of `filename` (src/ibp_api.cu:52-122).  The file is read by
+    `threads` threads (0: 16); bytes past its end are zero; a file larger than
+    the tensor raises (the reference copies past the tensor's end)."""
+    shape = tuple(int(d) for d in shape)
+    nbytes = torch.Size(shape).numel() * torch.empty(0, dtype=dtype).element_size()
+    storage = torch.UntypedStorage._new_shared(nbytes)
+    t = torch.empty(0, dtype=dtype).set_(storage, 0, shape)
+    got = ctypes.c_size_t(0)
+    _capi.call("ibp_read_file", str(filename).encode(), ctypes.c_void_p(t.data_ptr()) if nbytes else None, nbytes,
+               int(threads), ctypes.byref(got))
+    return t
 
 
 from .batched import BatchedFetch  # noqa: E402

@@ -23,6 +23,11 @@ class FetchTask(ctypes.Structure):
                 ("index_bytes", ctypes.c_int32), ("comp_len", ctypes.c_int32), ("comp_loc", ctypes.c_int32)]
 
 
+class Layout(ctypes.Structure):
+    """ibp_layout: condensed offsets and/or per-row mask rows (include/ibp.h)."""
+    _fields_ = [("offsets", _P), ("stream_bytes", _I64), ("clusters", _P), ("num_clusters", _I64)]
+
+
 class ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", _SZ), ("num_streams", _I32), ("max_batch_requests", _I32),
                 ("max_batch_rows", _I64)]
@@ -47,6 +52,13 @@ SIGNATURES = {
     "ibp_compress": (_I32, [_I32, _P, _P, _I64, _I64, _P, _P, _P, _I32, _I64, _P, _P]),
     "ibp_decompress_fetch": (_I32, [_I32, _P, _P, _I64, _I64, _P, _P, _P, _P, _I32, _I64, _I32, _I32, _I32,
                                     _I32, _I32, _P]),
+    "ibp_get_compress_size_ex": (_I32, [_I32, _P, _I64, _I64, _P, _P, _P, _I32, _I64, _P, _P,
+                                        ctypes.POINTER(Layout), _P]),
+    "ibp_condensed_offsets": (_I32, [_I32, _P, _I64, _I64, _P, _P, _P, _I32, _I64, _P, ctypes.POINTER(Layout), _P]),
+    "ibp_compress_ex": (_I32, [_I32, _P, _P, _I64, _I64, _P, _P, _P, _I32, _I64, _P, ctypes.POINTER(Layout), _P]),
+    "ibp_decompress_fetch_ex": (_I32, [_I32, _P, _P, _I64, _I64, _P, _P, _P, _P, _I32, _I64, _I32, _I32, _I32,
+                                       _I32, _I32, ctypes.POINTER(Layout), _P]),
+    "ibp_read_file": (_I32, [ctypes.c_char_p, _P, _SZ, _I32, ctypes.POINTER(_SZ)]),
     "ibp_set_fetch_tuning": (_I32, [_I32, _I32]),
     "ibp_plan_create": (_I32, [ctypes.POINTER(FetchTask), _I32, ctypes.POINTER(_P)]),
     "ibp_plan_destroy": (_I32, [_P]),

@@ -1,8 +1,8 @@
 // ibp_bits.h -- element-level bit arithmetic of the IBP slot codec.
 //
 // Compiled for the device (gfx950) and for the host (self-test).  T is
-// uint32_t or uint64_t (B = 32 or 64 bits); 1- and 2-byte elements only occur
-// in popcount/size work and use the 32-bit helpers on zero-extended values.
+// uint8_t, uint16_t, uint32_t or uint64_t (B = 8, 16, 32 or 64 bits).  Every
+// expression on a narrow T is cast back to T, since C++ promotes it to int.
 //
 // Shift semantics: the reference relies on PTX `shl`/`shr` returning 0 for
 // counts >= B (ibp_compress_dev.cuh:47,53-55,80; ibp_decompress_dev.cuh:315,
@@ -19,10 +19,21 @@ struct Bits {
     static constexpr int B = 8 * (int)sizeof(T);
 };
 
+__host__ __device__ __forceinline__ int popc(uint8_t v) { return __builtin_popcount((uint32_t)v); }
+__host__ __device__ __forceinline__ int popc(uint16_t v) { return __builtin_popcount((uint32_t)v); }
 __host__ __device__ __forceinline__ int popc(uint32_t v) { return __builtin_popcount(v); }
 __host__ __device__ __forceinline__ int popc(uint64_t v) { return __builtin_popcountll(v); }
 __host__ __device__ __forceinline__ int clz(uint32_t v) { return v ? __builtin_clz(v) : 32; }
 __host__ __device__ __forceinline__ int clz(uint64_t v) { return v ? __builtin_clzll(v) : 64; }
+__host__ __device__ __forceinline__ int clz(uint8_t v) { return clz((uint32_t)v) - 24; }
+__host__ __device__ __forceinline__ int clz(uint16_t v) { return clz((uint32_t)v) - 16; }
+
+// Unsigned type of a w-byte element.
+template <int W> struct UWord;
+template <> struct UWord<1> { using type = uint8_t; };
+template <> struct UWord<2> { using type = uint16_t; };
+template <> struct UWord<4> { using type = uint32_t; };
+template <> struct UWord<8> { using type = uint64_t; };
 
 // v << s and v >> s with the PTX clamp: counts >= B give 0.
 template <typename T>
@@ -39,7 +50,7 @@ __host__ __device__ __forceinline__ T shr(T v, int s)
 template <typename T>
 __host__ __device__ __forceinline__ T himask(int n)
 {
-    return n <= 0 ? T(0) : T(~T(0) << (Bits<T>::B - n));
+    return n <= 0 ? T(0) : T(T(~T(0)) << (Bits<T>::B - n));
 }
 
 // Gathers the bits of x at the set positions of `free` (the zero positions of

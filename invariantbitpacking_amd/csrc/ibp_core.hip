@@ -198,7 +198,9 @@ extern "C" int ibp_selftest_bitops(int trials, uint64_t seed)
 {
     clear_error();
     std::mt19937_64 rng(seed);
-    int st = selftest_width<uint32_t>(trials, rng);
-    if (st != IBP_OK) return st;
-    return selftest_width<uint64_t>(trials, rng);
+    int st = selftest_width<uint8_t>(trials, rng);
+    if (st == IBP_OK) st = selftest_width<uint16_t>(trials, rng);
+    if (st == IBP_OK) st = selftest_width<uint32_t>(trials, rng);
+    if (st == IBP_OK) st = selftest_width<uint64_t>(trials, rng);
+    return st;
 }

@@ -24,12 +24,22 @@ __global__ __launch_bounds__(256) void fetch_small_kernel(FetchParams p)
     fetch_small_range<T, IdxT, ROWS>(p, col, wave * ROWS, p.n, nwaves * ROWS);
 }
 
-template <typename T, typename IdxT>
+// Rows of <= 64 elements in the condensed and/or clustered layouts.
+template <typename T, typename IdxT, int ROWS, int LAYOUT>
+__global__ __launch_bounds__(256) void fetch_small_meta_kernel(FetchParams p)
+{
+    const SmallCols<T> col = load_small_cols<T>(p.mask, p.bitval, (int)p.vec_size);
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
+    fetch_small_range_meta<T, IdxT, ROWS, LAYOUT>(p, col, wave * ROWS, p.n, nwaves * ROWS);
+}
+
+template <typename T, typename IdxT, int LAYOUT>
 __global__ __launch_bounds__(256) void fetch_large_kernel(FetchParams p)
 {
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
-    fetch_large_range<T, IdxT>(p, wave, p.n, nwaves);
+    fetch_large_range<T, IdxT, LAYOUT>(p, wave, p.n, nwaves);
 }
 
 // Transfer baselines of the reference (include/decompress/ibp_decompress_host.cuh:114-166):
@@ -73,7 +83,8 @@ __global__ __launch_bounds__(256) void fetch_baseline_kernel(FetchParams p, int 
         } else {
             T acc = T(0);
             for (int64_t k = lane; k < cl; k += kWave) acc ^= slot[k];
-            asm volatile("" : : "v"(acc)); // the reads are the point of this mode
+            const uint64_t keep = (uint64_t)acc;
+            asm volatile("" : : "v"(keep)); // the reads are the point of this mode
         }
     }
 }
@@ -85,10 +96,31 @@ namespace {
 std::atomic<int> g_rows_in_flight{8};
 std::atomic<int> g_blocks_per_cu{16};
 
-template <typename T, typename IdxT>
-int launch_typed(const FetchParams &p, int impl, int comp_len, int grid, int block, hipStream_t stream)
+template <typename T, typename IdxT, int LAYOUT>
+void launch_meta(const FetchParams &p, int grid, int block, hipStream_t stream)
 {
-    if (impl == 2 || impl == 3 || impl == 4) {
+    if (p.vec_size <= kWave) {
+        switch (g_rows_in_flight.load()) {
+        case 16: hipLaunchKernelGGL((fetch_small_meta_kernel<T, IdxT, 16, LAYOUT>), dim3(grid), dim3(block), 0, stream, p); break;
+        case 4: hipLaunchKernelGGL((fetch_small_meta_kernel<T, IdxT, 4, LAYOUT>), dim3(grid), dim3(block), 0, stream, p); break;
+        case 2: hipLaunchKernelGGL((fetch_small_meta_kernel<T, IdxT, 2, LAYOUT>), dim3(grid), dim3(block), 0, stream, p); break;
+        default: hipLaunchKernelGGL((fetch_small_meta_kernel<T, IdxT, 8, LAYOUT>), dim3(grid), dim3(block), 0, stream, p); break;
+        }
+    } else {
+        hipLaunchKernelGGL((fetch_large_kernel<T, IdxT, LAYOUT>), dim3(grid), dim3(block), 0, stream, p);
+    }
+}
+
+template <typename T, typename IdxT>
+int launch_typed(const FetchParams &p, int layout, int impl, int comp_len, int grid, int block, hipStream_t stream)
+{
+    if (layout == kLayoutCondensed) {
+        launch_meta<T, IdxT, kLayoutCondensed>(p, grid, block, stream);
+    } else if (layout == kLayoutClustered) {
+        launch_meta<T, IdxT, kLayoutClustered>(p, grid, block, stream);
+    } else if (layout == (kLayoutCondensed | kLayoutClustered)) {
+        launch_meta<T, IdxT, kLayoutCondensed | kLayoutClustered>(p, grid, block, stream);
+    } else if (impl == 2 || impl == 3 || impl == 4) {
         hipLaunchKernelGGL((fetch_baseline_kernel<T, IdxT>), dim3(grid), dim3(block), 0, stream, p, impl, comp_len);
     } else if (p.vec_size <= kWave) {
         switch (g_rows_in_flight.load()) {
@@ -98,16 +130,25 @@ int launch_typed(const FetchParams &p, int impl, int comp_len, int grid, int blo
         default: hipLaunchKernelGGL((fetch_small_kernel<T, IdxT, 8>), dim3(grid), dim3(block), 0, stream, p); break;
         }
     } else {
-        hipLaunchKernelGGL((fetch_large_kernel<T, IdxT>), dim3(grid), dim3(block), 0, stream, p);
+        hipLaunchKernelGGL((fetch_large_kernel<T, IdxT, kLayoutSlots>), dim3(grid), dim3(block), 0, stream, p);
     }
     IBP_LAUNCH_CHECK();
     return IBP_OK;
 }
 
+template <typename T>
+int launch_width(const FetchParams &p, int layout, int index_bytes, int impl, int comp_len, int grid, int block,
+                 hipStream_t stream)
+{
+    if (p.index == nullptr || index_bytes == 8)
+        return launch_typed<T, int64_t>(p, layout, impl, comp_len, grid, block, stream);
+    return launch_typed<T, int32_t>(p, layout, impl, comp_len, grid, block, stream);
+}
+
 } // namespace
 
 int fetch_launch(const FetchParams &p, int elem_bytes, int index_bytes, int impl, int comp_len, int nblks,
-                 int nthreads, hipStream_t stream)
+                 int nthreads, hipStream_t stream, int layout)
 {
     const DeviceInfo &dev = device_info();
     // Blocks are at most 256 threads (four waves); larger requests are capped.
@@ -118,13 +159,12 @@ int fetch_launch(const FetchParams &p, int elem_bytes, int index_bytes, int impl
     const int64_t cap = (int64_t)dev.num_cus * g_blocks_per_cu.load();
     if (nblks <= 0 && grid > cap) grid = cap;
     if (grid < 1) grid = 1;
-    const bool idx64 = index_bytes == 8;
-    if (elem_bytes == 8) {
-        if (p.index == nullptr || idx64) return launch_typed<uint64_t, int64_t>(p, impl, comp_len, (int)grid, block, stream);
-        return launch_typed<uint64_t, int32_t>(p, impl, comp_len, (int)grid, block, stream);
+    switch (elem_bytes) {
+    case 1: return launch_width<uint8_t>(p, layout, index_bytes, impl, comp_len, (int)grid, block, stream);
+    case 2: return launch_width<uint16_t>(p, layout, index_bytes, impl, comp_len, (int)grid, block, stream);
+    case 4: return launch_width<uint32_t>(p, layout, index_bytes, impl, comp_len, (int)grid, block, stream);
+    default: return launch_width<uint64_t>(p, layout, index_bytes, impl, comp_len, (int)grid, block, stream);
     }
-    if (p.index == nullptr || idx64) return launch_typed<uint32_t, int64_t>(p, impl, comp_len, (int)grid, block, stream);
-    return launch_typed<uint32_t, int32_t>(p, impl, comp_len, (int)grid, block, stream);
 }
 
 // Words of the speculative first read of a packed small row: the expected
@@ -142,20 +182,28 @@ int first_words_for(int64_t vec_size, int elem_bytes, int comp_len)
 
 using namespace ibp;
 
-extern "C" int ibp_decompress_fetch(int elem_bytes, void *out, const void *comp, int64_t num_vecs,
-                                    int64_t vec_size, const void *mask, const void *bitval,
-                                    const int32_t *bitmask, const void *index, int index_bytes, int64_t n,
-                                    int comp_len, int nblks, int nthreads, int impl, int comp_loc, void *stream)
+extern "C" int ibp_decompress_fetch_ex(int elem_bytes, void *out, const void *comp, int64_t num_vecs,
+                                       int64_t vec_size, const void *mask, const void *bitval,
+                                       const int32_t *bitmask, const void *index, int index_bytes, int64_t n,
+                                       int comp_len, int nblks, int nthreads, int impl, int comp_loc,
+                                       const ibp_layout *layout, void *stream)
 {
     (void)comp_loc;
-    IBP_CHECK(elem_bytes == 4 || elem_bytes == 8, "Input tensor must be 4-byte or 8-byte datatype (got %d)", elem_bytes);
+    IBP_CHECK(elem_bytes == 1 || elem_bytes == 2 || elem_bytes == 4 || elem_bytes == 8,
+              "Input tensor must be 1, 2, 4 or 8-byte datatype (got %d)", elem_bytes);
     IBP_CHECK(vec_size > 0 && num_vecs >= 0 && n >= 0, "invalid shape num_vecs=%lld vec_size=%lld n=%lld",
               (long long)num_vecs, (long long)vec_size, (long long)n);
     IBP_CHECK(index == nullptr || index_bytes == 4 || index_bytes == 8, "index must be int32 or int64");
     IBP_CHECK(impl >= -1 && impl <= 5, "unknown impl %d", impl);
     IBP_CHECK(nthreads <= 0 || (nthreads % 64 == 0 && nthreads <= 1024), "nthreads must be a multiple of 64 up to 1024");
+    const bool condensed = layout && layout->offsets;
+    const bool clustered = layout && layout->clusters;
+    IBP_CHECK(!clustered || layout->num_clusters >= 1, "num_clusters must be >= 1 with clusters");
+    IBP_CHECK(!condensed || layout->stream_bytes >= 0, "stream_bytes must be >= 0");
+    IBP_CHECK(!(condensed || clustered) || (impl != 2 && impl != 3 && impl != 4),
+              "impl %d (transfer baseline) takes the slot layout without clusters", impl);
     if (n == 0) return IBP_OK;
-    IBP_CHECK(out && comp && mask && bitval && bitmask, "null pointer argument");
+    IBP_CHECK(out && comp && mask && bitval && (bitmask || condensed), "null pointer argument");
     IBP_CHECK(index != nullptr || n <= num_vecs, "without an index n (%lld) must not exceed num_vecs (%lld)",
               (long long)n, (long long)num_vecs);
     FetchParams p;
@@ -170,9 +218,29 @@ extern "C" int ibp_decompress_fetch(int elem_bytes, void *out, const void *comp,
     p.n = n;
     p.first_words = first_words_for(vec_size, elem_bytes, comp_len);
     p.pad = 0;
-    p.err = index ? index_error_counter() : nullptr;
+    p.err = (index || condensed || clustered) ? index_error_counter() : nullptr;
+    int lay = kLayoutSlots;
+    if (condensed) {
+        lay |= kLayoutCondensed;
+        p.offsets = layout->offsets;
+        p.stream_bytes = layout->stream_bytes;
+    }
+    if (clustered) {
+        lay |= kLayoutClustered;
+        p.clusters = layout->clusters;
+        p.num_clusters = layout->num_clusters;
+    }
     const int cl = comp_len > 0 ? comp_len : (int)vec_size;
-    return fetch_launch(p, elem_bytes, index_bytes, impl, cl, nblks, nthreads, as_stream(stream));
+    return fetch_launch(p, elem_bytes, index_bytes, impl, cl, nblks, nthreads, as_stream(stream), lay);
+}
+
+extern "C" int ibp_decompress_fetch(int elem_bytes, void *out, const void *comp, int64_t num_vecs,
+                                    int64_t vec_size, const void *mask, const void *bitval,
+                                    const int32_t *bitmask, const void *index, int index_bytes, int64_t n,
+                                    int comp_len, int nblks, int nthreads, int impl, int comp_loc, void *stream)
+{
+    return ibp_decompress_fetch_ex(elem_bytes, out, comp, num_vecs, vec_size, mask, bitval, bitmask, index,
+                                   index_bytes, n, comp_len, nblks, nthreads, impl, comp_loc, nullptr, stream);
 }
 
 extern "C" int ibp_set_fetch_tuning(int rows_in_flight, int blocks_per_cu)

@@ -9,6 +9,17 @@
 // B - popc(mask[j]) free bits per flagged element and B raw bits otherwise;
 // the rest of the slot is zero.  A row whose bitmask bit is clear is stored raw.
 //
+// Row layouts (template parameter LAYOUT, bit flags):
+//  * slots (0): stored row s is the V-element slot at element s*V; packed
+//    when bitmask bit s is set (compress_inplace, src/compress.cu:176-236);
+//  * condensed (kLayoutCondensed): stored row s is bytes [offsets[s-1],
+//    offsets[s]) of a stream (offsets[-1] = 0), the slot cut at its size;
+//    packed when that length is not V*w (compress_condensed,
+//    src/compress.cu:238-313, ibp_compress_kernel.cuh:59-94);
+//  * clustered (kLayoutClustered): row s is coded with mask/bitval row
+//    clusters[s] of a [K x V] table (the reference applies row 0 to every
+//    row, src/compress.cu:221-232).
+//
 // Mapping: one wave64 per row.  Lane j owns element j of each 64-element chunk.
 // For w = 8 the 64-bit header word c holds exactly the flags of chunk c.
 #pragma once
@@ -20,19 +31,25 @@
 
 namespace ibp {
 
+enum : int { kLayoutSlots = 0, kLayoutCondensed = 1, kLayoutClustered = 2 };
+
 struct FetchParams {
-    void *out;
-    const void *comp;
-    const void *mask;
-    const void *bitval;
-    const uint32_t *bitmask;
-    const void *index; // nullptr: identity
-    int64_t num_vecs;
-    int64_t vec_size;
-    int64_t n;
-    int32_t first_words; // words read before the header is known (small rows)
-    int32_t pad;
-    unsigned long long *err;
+    void *out = nullptr;
+    const void *comp = nullptr;
+    const void *mask = nullptr;
+    const void *bitval = nullptr;
+    const uint32_t *bitmask = nullptr; // slots layout
+    const void *index = nullptr;       // nullptr: identity
+    int64_t num_vecs = 0;              // stored rows
+    int64_t vec_size = 0;
+    int64_t n = 0;
+    int32_t first_words = 0; // words read before the header is known (small rows)
+    int32_t pad = 0;
+    unsigned long long *err = nullptr;
+    const int64_t *offsets = nullptr;  // condensed layout
+    const int32_t *clusters = nullptr; // clustered layout
+    int64_t stream_bytes = 0;          // condensed: bytes of the stream (reads stay inside)
+    int64_t num_clusters = 1;          // clustered: rows of mask/bitval
 };
 
 // Fetched rows are written once and not read back by the kernel: streaming
@@ -126,11 +143,78 @@ __device__ __forceinline__ bool small_flag(T w, int lane)
     if constexpr (sizeof(T) == 8) {
         const uint64_t h = readlane((uint64_t)w, 0);
         return (h >> lane) & 1ull;
-    } else {
+    } else if constexpr (sizeof(T) == 4) {
         const uint32_t h0 = readlane((uint32_t)w, 0);
         const uint32_t h1 = readlane((uint32_t)w, 1);
         return ((lane < 32 ? h0 : h1) >> (lane & 31)) & 1u;
+    } else { // 8- or 16-bit words: flag j is bit j % B of header word j / B
+        const T h = shfl(w, lane / Bits<T>::B);
+        return (h >> (lane % Bits<T>::B)) & 1;
     }
+}
+
+// Per-row metadata of the condensed and clustered layouts, as loaded (raw):
+// decoded only where used, so the loads of the next group stay in flight.
+struct RowMetaRaw {
+    int64_t lo, hi; // condensed: byte range of the stored row
+    int32_t cl;     // clustered: mask row
+    uint32_t bm;    // slots: bitmask word of the row
+};
+
+template <int LAYOUT>
+__device__ __forceinline__ RowMetaRaw load_meta(const FetchParams &p, int64_t s, bool live)
+{
+    RowMetaRaw m{0, 0, 0, 0u};
+    if (live && s >= 0 && s < p.num_vecs) {
+        if constexpr ((LAYOUT & kLayoutCondensed) != 0) {
+            m.lo = s > 0 ? p.offsets[s - 1] : 0;
+            m.hi = p.offsets[s];
+        } else {
+            m.bm = p.bitmask[s >> 5];
+        }
+        if constexpr ((LAYOUT & kLayoutClustered) != 0) m.cl = p.clusters[s];
+    }
+    return m;
+}
+
+struct RowMeta {
+    int64_t word; // first element of the stored row in comp
+    int64_t len;  // elements stored (V for a raw row)
+    int32_t cl;   // mask row
+    bool ok;      // index in range, byte range inside the stream, cluster in range
+    bool packed;
+};
+
+template <typename T, int LAYOUT>
+__device__ __forceinline__ RowMeta decode_meta(const FetchParams &p, int64_t s, bool live, const RowMetaRaw &r)
+{
+    constexpr int64_t W = (int64_t)sizeof(T);
+    const int64_t V = p.vec_size;
+    RowMeta m{0, 0, 0, false, false};
+    bool ok = live && s >= 0 && s < p.num_vecs;
+    if constexpr ((LAYOUT & kLayoutCondensed) != 0) {
+        const int64_t len = r.hi - r.lo;
+        ok = ok && r.lo >= 0 && len >= 0 && r.hi <= p.stream_bytes && len <= V * W && (r.lo % W) == 0 &&
+             (len % W) == 0;
+        m.word = r.lo / W;
+        m.len = len / W;
+        m.packed = len != V * W;
+    } else {
+        m.word = s * V;
+        m.len = V;
+        m.packed = (r.bm >> (s & 31)) & 1u;
+    }
+    if constexpr ((LAYOUT & kLayoutClustered) != 0) {
+        ok = ok && r.cl >= 0 && r.cl < p.num_clusters;
+        m.cl = r.cl;
+    }
+    m.ok = ok;
+    if (!ok) {
+        m.word = 0;
+        m.len = 0;
+        m.packed = false;
+    }
+    return m;
 }
 
 // Fetches and unpacks rows of a table whose rows have at most 64 elements:
@@ -230,9 +314,130 @@ __device__ __forceinline__ void fetch_small_range(const FetchParams &p, const Sm
     }
 }
 
-// Fetches and unpacks output row i (source row s, already loaded) of a table
+// fetch_small_range for the condensed and clustered layouts.  A row's
+// metadata (its byte range, its mask row) depends on its index, so indices
+// are loaded two groups ahead and metadata one group ahead; with a mask row
+// per row the column constants are loaded per row and the generic deposit is
+// used.  Rows whose index, byte range or mask row is invalid give zero rows
+// and are counted.  Wave-uniform.
+template <typename T, typename IdxT, int ROWS, int LAYOUT>
+__device__ __forceinline__ void fetch_small_range_meta(const FetchParams &p, const SmallCols<T> &col,
+                                                       int64_t begin, int64_t end, int64_t step)
+{
+    constexpr int B = Bits<T>::B;
+    constexpr bool CLUS = (LAYOUT & kLayoutClustered) != 0;
+    const int lane = lane_id();
+    const int V = (int)p.vec_size;
+    const int Hw = (int)(header_bytes(V, (int)sizeof(T)) / (int)sizeof(T));
+    const int E = p.first_words;
+    const T *comp = (const T *)p.comp;
+    const T *mask = (const T *)p.mask;
+    const T *bitval = (const T *)p.bitval;
+    T *out = (T *)p.out;
+    const bool colv = lane < V;
+
+    auto live_at = [&](int64_t b) { return lane < ROWS && b + lane < end; };
+    auto id_at = [&](int64_t b) -> int64_t { return live_at(b) ? load_index<IdxT>(p.index, b + lane) : 0; };
+
+    int64_t cur = id_at(begin);
+    RowMetaRaw craw = load_meta<LAYOUT>(p, cur, live_at(begin));
+    int64_t nxt = id_at(begin + step);
+    for (int64_t base = begin; base < end; base += step) {
+        const bool live = live_at(base);
+        const RowMeta me = decode_meta<T, LAYOUT>(p, cur, live, craw);
+        const uint64_t okmask = ballot(me.ok);
+        const uint64_t pkmask = ballot(me.ok && me.packed);
+
+        // Stage 1: stored-row prefixes (and mask rows), the next group's
+        // metadata, the index of the group after it.
+        int64_t word[ROWS];
+        int len[ROWS];
+        T w[ROWS], cm[ROWS], cb[ROWS];
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r) {
+            const bool ok = (okmask >> r) & 1ull;
+            word[r] = readlane(me.word, r);
+            len[r] = (int)readlane(me.len, r);
+            const int first = E < len[r] ? E : len[r];
+            w[r] = T(0);
+            if (ok && lane < first) w[r] = comp[word[r] + lane];
+            cm[r] = cb[r] = T(0);
+            if constexpr (CLUS) {
+                const int64_t mrow = (int64_t)readlane(me.cl, r) * V;
+                if (ok && colv) {
+                    cm[r] = mask[mrow + lane];
+                    cb[r] = bitval[mrow + lane];
+                }
+            }
+        }
+        const RowMetaRaw nraw = load_meta<LAYOUT>(p, nxt, live_at(base + step));
+        const int64_t nxt2 = id_at(base + 2 * step);
+
+        // Stage 2: header -> flags -> bit offsets -> words needed; read the rest.
+        int pos[ROWS];
+        bool flag[ROWS];
+        int need[ROWS];
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r) {
+            const bool ok = (okmask >> r) & 1ull;
+            const bool packed = (pkmask >> r) & 1ull;
+            flag[r] = false;
+            pos[r] = 0;
+            need[r] = ok ? len[r] : 0;
+            if (packed) { // wave-uniform
+                const bool f = small_flag<T>(w[r], lane);
+                flag[r] = colv && f;
+                int lf;
+                if constexpr (CLUS)
+                    lf = B - popc(cm[r]);
+                else
+                    lf = col.len_flagged;
+                const int L = colv ? (flag[r] ? lf : B) : 0;
+                const int incl = wave_incl_scan(L);
+                const int total = readlane(incl, kWave - 1);
+                pos[r] = incl - L;
+                const int nw = Hw + (total + B - 1) / B;
+                need[r] = nw < len[r] ? nw : len[r];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r)
+            if (need[r] > E && lane >= E && lane < need[r]) w[r] = comp[word[r] + lane];
+
+        // Stage 3: unpack and store.
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r) {
+            const int64_t i = base + r;
+            const bool ok = (okmask >> r) & 1ull;
+            const bool packed = (pkmask >> r) & 1ull;
+            T val = w[r];
+            if (packed) {
+                const int q = Hw + pos[r] / B;
+                const int o = pos[r] % B;
+                const T w0 = shfl(w[r], q);
+                const T w1 = shfl(w[r], q + 1 < kWave ? q + 1 : kWave - 1);
+                const T v = stream_bits<T>(w0, w1, o);
+                if constexpr (CLUS)
+                    val = flag[r] ? T(cb[r] | deposit_msb<T>(v, T(~cm[r]))) : v;
+                else
+                    val = flag[r] ? T(col.bitval | small_deposit<T>(col, v)) : v;
+            }
+            if (i < end && lane < V) store_row_word<T>(&out[i * V + lane], ok ? val : T(0));
+        }
+        const uint64_t livemask = ballot(live);
+        if (p.err && lane == 0) {
+            const int bad = __builtin_popcountll(livemask & ~okmask);
+            if (bad) atomicAdd(p.err, (unsigned long long)bad);
+        }
+        cur = nxt;
+        craw = nraw;
+        nxt = nxt2;
+    }
+}
+
+// Fetches and unpacks output row i (stored row s, already loaded) of a table
 // with rows of any length, one 64-element chunk at a time.  Wave-uniform.
-template <typename T>
+template <typename T, int LAYOUT>
 __device__ __forceinline__ void fetch_large_row_from(const FetchParams &p, int64_t i, int64_t s)
 {
     constexpr int B = Bits<T>::B;
@@ -240,18 +445,18 @@ __device__ __forceinline__ void fetch_large_row_from(const FetchParams &p, int64
     const int lane = lane_id();
     const int64_t V = p.vec_size;
     const int64_t Hw = header_bytes(V, (int)sizeof(T)) / (int64_t)sizeof(T);
-    const T *mask = (const T *)p.mask;
-    const T *bitval = (const T *)p.bitval;
     T *dst = (T *)p.out + i * V;
 
-    if (s < 0 || s >= p.num_vecs) {
+    const RowMeta me = decode_meta<T, LAYOUT>(p, s, true, load_meta<LAYOUT>(p, s, true));
+    if (!me.ok) {
         for (int64_t k = lane; k < V; k += kWave) dst[k] = T(0);
         if (p.err && lane == 0) atomicAdd(p.err, 1ull);
         return;
     }
-    const T *slot = (const T *)p.comp + s * V;
-    const bool packed = (p.bitmask[s >> 5] >> (s & 31)) & 1u;
-    if (!packed) {
+    const T *slot = (const T *)p.comp + me.word;
+    const T *mask = (const T *)p.mask + (int64_t)me.cl * V;
+    const T *bitval = (const T *)p.bitval + (int64_t)me.cl * V;
+    if (!me.packed) {
         // Raw row: four independent loads in flight per lane.
         for (int64_t k0 = lane; k0 < V; k0 += 4 * kWave) {
             T a[4];
@@ -268,6 +473,7 @@ __device__ __forceinline__ void fetch_large_row_from(const FetchParams &p, int64
         }
         return;
     }
+    const int64_t hlim = Hw < me.len ? Hw : me.len;
     const T *payload = slot + Hw;
     int64_t hdr_group = -1;
     T hdr = T(0);
@@ -284,15 +490,17 @@ __device__ __forceinline__ void fetch_large_row_from(const FetchParams &p, int64
         if (whi / kWave != hdr_group) { // header words are staged 64 at a time
             hdr_group = whi / kWave;
             const int64_t hk = hdr_group * kWave + lane;
-            hdr = hk < Hw ? slot[hk] : T(0);
+            hdr = hk < hlim ? slot[hk] : T(0);
         }
         T hw;
-        if (wlo == whi) {
+        if constexpr (sizeof(T) == 8) {
             hw = readlane(hdr, (int)(wlo % kWave));
-        } else { // w = 4: two header words per chunk
+        } else if constexpr (sizeof(T) == 4) { // two header words per chunk
             const T h0 = readlane(hdr, (int)(wlo % kWave));
             const T h1 = readlane(hdr, (int)(whi % kWave));
             hw = lane < FLAGS_PER_WORD ? h0 : h1;
+        } else { // 4 or 8 header words per chunk, all in the staged group
+            hw = shfl(hdr, (int)(((c * kWave + lane) / FLAGS_PER_WORD) % kWave));
         }
         const bool f = valid && ((hw >> (j % FLAGS_PER_WORD)) & 1);
         const int L = valid ? (f ? B - popc(m) : B) : 0;
@@ -301,8 +509,8 @@ __device__ __forceinline__ void fetch_large_row_from(const FetchParams &p, int64
         const int64_t p_bit = cursor + incl - L;
         const int64_t w0 = cursor / B;
         const int64_t cnt = (cursor + tot + B - 1) / B - w0; // <= 65
-        // Reads stay inside the slot even for a corrupt header.
-        const int64_t room = V - Hw - w0;
+        // Reads stay inside the stored row even for a corrupt header.
+        const int64_t room = me.len - Hw - w0;
         const T A = (lane < cnt && lane < room) ? payload[w0 + lane] : T(0);
         const T X = (cnt > kWave && kWave < room) ? payload[w0 + kWave] : T(0);
         const int q = (int)(p_bit / B - w0);
@@ -320,14 +528,14 @@ __device__ __forceinline__ void fetch_large_row_from(const FetchParams &p, int64
 
 // Output rows [b, e), step `step`, of a wide-row table.  The next row's
 // index is loaded while the current row is unpacked.
-template <typename T, typename IdxT>
+template <typename T, typename IdxT, int LAYOUT = kLayoutSlots>
 __device__ __forceinline__ void fetch_large_range(const FetchParams &p, int64_t b, int64_t e, int64_t step)
 {
     if (b >= e) return;
     int64_t s = readfirstlane(load_index<IdxT>(p.index, b));
     for (int64_t i = b; i < e; i += step) {
         const int64_t nxt = i + step < e ? readfirstlane(load_index<IdxT>(p.index, i + step)) : 0;
-        fetch_large_row_from<T>(p, i, s);
+        fetch_large_row_from<T, LAYOUT>(p, i, s);
         s = nxt;
     }
 }

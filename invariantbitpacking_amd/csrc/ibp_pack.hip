@@ -1,16 +1,20 @@
-// ibp_pack.hip -- compressed-size scan and the (in-place) pack.
+// ibp_pack.hip -- compressed sizes, the in-place / slot pack and the condensed
+// stream.
 //
-// get_compress_size: src/compress.cu:25-174 -> check_compress_size_kernel
+// get_compress_size:  src/compress.cu:25-174 -> check_compress_size_kernel
 //   (include/misc/ibp_misc_kernels.cuh:9-32, ibp_misc_dev.cuh:20-46).
-// compress_inplace:  src/compress.cu:176-236 -> compress_inplace_kernel
+// compress_inplace:   src/compress.cu:176-236 -> compress_inplace_kernel
 //   (include/compress/ibp_compress_kernel.cuh:10-56, ibp_compress_dev.cuh:8-97).
+// compress_condensed: src/compress.cu:238-313 -> sizes, thrust::inclusive_scan,
+//   compress_condensed_kernel (ibp_compress_kernel.cuh:59-94).
 //
 // MI355X mapping: one wave64 per row, lane j owns element j of each 64-element
 // chunk; a wave owns 32 consecutive output rows so it writes their bitmask
 // word itself (no atomics, no pre-zeroing).  The packed slot is assembled in a
-// per-wave LDS buffer (64-bit ds_or for the payload, ballot for the header
-// words) and then written out whole, so in-place packing never overwrites
-// input the wave still has to read.
+// per-wave LDS buffer (ds_or for the payload, ballot for the header words)
+// and then written out, so in-place packing never overwrites input the wave
+// still has to read.  Words of 1 or 2 bytes are OR-ed through the 32-bit word
+// that contains them (LDS has no sub-dword atomics).
 #include <type_traits>
 
 #include "ibp_bits.h"
@@ -38,9 +42,18 @@ __device__ __forceinline__ int row_saved_bits(const T *row, const T *mask, const
     return wave_sum(saved);
 }
 
+// Mask row of source row r (clusters NULL: row 0); -1 if the id is invalid.
+__device__ __forceinline__ int64_t mask_row_of(const int32_t *clusters, int64_t num_clusters, int64_t r)
+{
+    if (!clusters) return 0;
+    const int64_t c = readfirstlane((int)clusters[r]);
+    return (c >= 0 && c < num_clusters) ? c : -1;
+}
+
 template <typename T, typename IdxT>
 __global__ __launch_bounds__(256) void size_kernel(const T *data, int64_t num_vecs, int64_t V, const T *mask,
-                                                   const T *bitval, const void *index, int64_t n,
+                                                   const T *bitval, const int32_t *clusters,
+                                                   int64_t num_clusters, const void *index, int64_t n,
                                                    int64_t *sizes, unsigned long long *total,
                                                    unsigned long long *err)
 {
@@ -51,8 +64,9 @@ __global__ __launch_bounds__(256) void size_kernel(const T *data, int64_t num_ve
     for (int64_t i = wave; i < n; i += nwaves) {
         const int64_t r = readfirstlane(row_of<IdxT>(index, i));
         int64_t size = 0;
-        if (r >= 0 && r < num_vecs) {
-            const int saved = row_saved_bits<T>(data + r * V, mask, bitval, V);
+        const int64_t c = (r >= 0 && r < num_vecs) ? mask_row_of(clusters, num_clusters, r) : -1;
+        if (c >= 0) {
+            const int saved = row_saved_bits<T>(data + r * V, mask + c * V, bitval + c * V, V);
             size = slot_size(V, (int)sizeof(T), saved);
         } else {
             ++bad;
@@ -77,46 +91,104 @@ __device__ __forceinline__ void slot_fence()
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
 }
 
+// buf[q] |= v.  Words narrower than 32 bits are OR-ed through their 32-bit
+// container (little-endian: byte b of the container is bits 8b..8b+7); the
+// buffer is 4-byte aligned and padded to a multiple of 16 bytes.
+template <typename T>
+__device__ __forceinline__ void or_word(T *buf, int64_t q, T v)
+{
+    if constexpr (sizeof(T) >= 4) {
+        __hip_atomic_fetch_or(&buf[q], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+        const int64_t byte = q * (int64_t)sizeof(T);
+        uint32_t *d = (uint32_t *)((unsigned char *)buf + (byte & ~int64_t(3)));
+        __hip_atomic_fetch_or(d, (uint32_t)v << (8 * (int)(byte & 3)), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// Bytes of one wave's slot buffer: V words padded to 16 bytes.
+__host__ __device__ __forceinline__ int64_t slot_buf_bytes(int64_t V, int w) { return (V * w + 15) / 16 * 16; }
+
+struct PackParams {
+    const void *in;
+    void *out;
+    int64_t num_vecs;
+    int64_t V;
+    const void *mask;
+    const void *bitval;
+    const int32_t *clusters; // mask row per source row, or nullptr
+    int64_t num_clusters;
+    const int64_t *offsets;  // condensed output: inclusive end byte offset per output row
+    int64_t stream_bytes;
+    const void *index;
+    int64_t n;
+    uint32_t *bitmask; // may be nullptr with a condensed output
+    void *gbuf;        // per-wave slot buffers when they do not fit LDS
+    unsigned long long *err;
+};
+
 // Pack kernel.  BUF_LDS: the per-wave slot buffer is in LDS (dynamic shared
-// memory, V words per wave); otherwise it is `gbuf` in device memory.
-template <typename T, typename IdxT, bool BUF_LDS>
-__global__ __launch_bounds__(256) void pack_kernel(const T *in, T *out, int64_t num_vecs, int64_t V,
-                                                   const T *mask, const T *bitval, const void *index,
-                                                   int64_t n, uint32_t *bitmask, T *gbuf,
-                                                   unsigned long long *err)
+// memory); otherwise in p.gbuf.  CONDENSED: output row i goes to bytes
+// [offsets[i-1], offsets[i]) of the stream, cut at its size; otherwise to the
+// V-element slot i.
+template <typename T, typename IdxT, bool BUF_LDS, bool CONDENSED>
+__global__ __launch_bounds__(256) void pack_kernel(PackParams p)
 {
     constexpr int B = Bits<T>::B;
+    constexpr int W = (int)sizeof(T);
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const int lane = lane_id();
     const int wave_in_block = threadIdx.x / kWave;
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
-    T *buf = BUF_LDS ? (T *)lds_raw + (int64_t)wave_in_block * V : gbuf + wave * V;
-    const int64_t Hw = header_bytes(V, (int)sizeof(T)) / (int64_t)sizeof(T);
-    const int64_t vec_bytes = V * (int64_t)sizeof(T);
-    const bool in_place = (const void *)in == (const void *)out && index == nullptr;
-    const int64_t ngroups = (n + 31) / 32;
+    const int64_t V = p.V;
+    const int64_t buf_bytes = slot_buf_bytes(V, W);
+    unsigned char *bufb = BUF_LDS ? lds_raw + (int64_t)wave_in_block * buf_bytes
+                                  : (unsigned char *)p.gbuf + wave * buf_bytes;
+    T *buf = (T *)bufb;
+    const T *in = (const T *)p.in;
+    const int64_t Hw = header_bytes(V, W) / W;
+    const int64_t vec_bytes = V * W;
+    const bool in_place = !CONDENSED && p.in == (const void *)p.out && p.index == nullptr;
+    const int64_t ngroups = (p.n + 31) / 32;
     unsigned long long bad = 0;
 
     for (int64_t g = wave; g < ngroups; g += nwaves) {
         uint32_t bits = 0;
-        const int64_t iend = (g + 1) * 32 < n ? (g + 1) * 32 : n;
+        const int64_t iend = (g + 1) * 32 < p.n ? (g + 1) * 32 : p.n;
         for (int64_t i = g * 32; i < iend; ++i) {
-            const int64_t r = readfirstlane(row_of<IdxT>(index, i));
-            if (r < 0 || r >= num_vecs) {
+            const int64_t r = readfirstlane(row_of<IdxT>(p.index, i));
+            const int64_t c = (r >= 0 && r < p.num_vecs) ? mask_row_of(p.clusters, p.num_clusters, r) : -1;
+            if (c < 0) {
                 ++bad;
                 continue;
             }
             const T *src = in + r * V;
-            T *dst = out + i * V;
+            const T *mask = (const T *)p.mask + c * V;
+            const T *bitval = (const T *)p.bitval + c * V;
             // Size test (ibp_compress_kernel.cuh:28-31): pack only if smaller.
             const int saved = row_saved_bits<T>(src, mask, bitval, V);
-            if (slot_size(V, (int)sizeof(T), saved) >= vec_bytes) {
+            const int64_t size = slot_size(V, W, saved);
+            T *dst;
+            if constexpr (CONDENSED) {
+                const int64_t lo = i > 0 ? readfirstlane(p.offsets[i - 1]) : 0;
+                const int64_t hi = readfirstlane(p.offsets[i]);
+                // The stored length must be this row's size: offsets from
+                // another dataset or mask never make a write leave the row.
+                if (lo < 0 || hi - lo != size || hi > p.stream_bytes || (lo % W) != 0) {
+                    ++bad;
+                    continue;
+                }
+                dst = (T *)((unsigned char *)p.out + lo);
+            } else {
+                dst = (T *)p.out + i * V;
+            }
+            if (size >= vec_bytes) {
                 if (!in_place)
                     for (int64_t k = lane; k < V; k += kWave) dst[k] = src[k];
                 continue;
             }
-            for (int64_t k = lane; k < V; k += kWave) buf[k] = T(0);
+            for (int64_t k = lane; k < buf_bytes / 4; k += kWave) ((uint32_t *)bufb)[k] = 0u;
             slot_fence<BUF_LDS>();
             int64_t cursor = 0;
             for (int64_t c0 = 0; c0 < V; c0 += kWave) {
@@ -125,13 +197,12 @@ __global__ __launch_bounds__(256) void pack_kernel(const T *in, T *out, int64_t 
                 const T x = valid ? src[j] : T(0);
                 const T m = valid ? mask[j] : T(0);
                 const bool f = valid && (T)(x & m) == (valid ? bitval[j] : T(0));
-                // Header flags of this chunk, LSB first (ibp_compress_dev.cuh:60-71).
+                // Header flags of this chunk, LSB first (ibp_compress_dev.cuh:60-71):
+                // 64 / B header words, word k holding flags k*B .. k*B+B-1.
                 const uint64_t hb = ballot(f);
-                if constexpr (sizeof(T) == 8) {
-                    if (lane == 0) buf[c0 / 64] = (T)hb;
-                } else {
-                    if (lane == 0) buf[c0 / 32] = (T)(uint32_t)hb;
-                    if (lane == 1 && c0 + 32 < V) buf[c0 / 32 + 1] = (T)(uint32_t)(hb >> 32);
+                if (lane < kWave / B && c0 + lane * B < V) {
+                    const T hw = (T)(hb >> (lane * B));
+                    if (hw) or_word<T>(buf, c0 / B + lane, hw);
                 }
                 const int L = valid ? (f ? B - popc(m) : B) : 0;
                 const int incl = wave_incl_scan(L);
@@ -142,68 +213,165 @@ __global__ __launch_bounds__(256) void pack_kernel(const T *in, T *out, int64_t 
                 const int64_t q = Hw + pbit / B;
                 const int o = (int)(pbit % B);
                 if (L > 0) {
-                    __hip_atomic_fetch_or(&buf[q], shr<T>(v, o), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    if (o + L > B)
-                        __hip_atomic_fetch_or(&buf[q + 1], shl<T>(v, B - o), __ATOMIC_RELAXED,
-                                              __HIP_MEMORY_SCOPE_AGENT);
+                    or_word<T>(buf, q, shr<T>(v, o));
+                    if (o + L > B) or_word<T>(buf, q + 1, shl<T>(v, B - o));
                 }
                 cursor += tot;
             }
             slot_fence<BUF_LDS>();
-            for (int64_t k = lane; k < V; k += kWave)
+            const int64_t words = size / W; // the rest of the slot is zero
+            for (int64_t k = lane; k < (CONDENSED ? words : V); k += kWave)
                 dst[k] = __hip_atomic_load(&buf[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             bits |= 1u << (i - g * 32);
         }
-        if (lane == 0) bitmask[g] = bits;
+        if (lane == 0 && p.bitmask) p.bitmask[g] = bits;
     }
-    if (err && bad && lane == 0) atomicAdd(err, bad);
+    if (p.err && bad && lane == 0) atomicAdd(p.err, bad);
+}
+
+// ---- inclusive scan of int64 (the thrust::inclusive_scan of compress_condensed,
+// src/compress.cu:284-285): tile sums, an exclusive scan of the tile sums in
+// one block, then each tile scanned with its carry-in.
+
+constexpr int kScanPer = 16;                  // consecutive values per thread
+constexpr int kScanTile = 256 * kScanPer;     // values per 256-thread block
+
+// Inclusive scan over the block's threads (blockDim.x <= 1024) through LDS.
+__device__ __forceinline__ int64_t block_incl_scan(int64_t x, int64_t *lds)
+{
+    const int t = threadIdx.x;
+    lds[t] = x;
+    __syncthreads();
+    for (int d = 1; d < (int)blockDim.x; d <<= 1) {
+        const int64_t y = t >= d ? lds[t - d] : 0;
+        __syncthreads();
+        x += y;
+        lds[t] = x;
+        __syncthreads();
+    }
+    return x;
+}
+
+__global__ __launch_bounds__(256) void tile_sum_kernel(const int64_t *v, int64_t n, int64_t *sums)
+{
+    __shared__ int64_t lds[256];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanPer;
+    int64_t acc = 0;
+    for (int k = 0; k < kScanPer; ++k)
+        if (base + k < n) acc += v[base + k];
+    const int64_t tot = block_incl_scan(acc, lds);
+    if (threadIdx.x == blockDim.x - 1) sums[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(1024) void scan_sums_kernel(int64_t *sums, int64_t ntiles)
+{
+    __shared__ int64_t lds[1024];
+    int64_t carry = 0;
+    for (int64_t b = 0; b < ntiles; b += blockDim.x) {
+        const int64_t k = b + threadIdx.x;
+        const int64_t x = k < ntiles ? sums[k] : 0;
+        const int64_t incl = block_incl_scan(x, lds);
+        if (k < ntiles) sums[k] = carry + incl - x; // exclusive
+        carry += lds[blockDim.x - 1];
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void tile_scan_kernel(int64_t *v, int64_t n, const int64_t *carry_in)
+{
+    __shared__ int64_t lds[256];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanPer;
+    int64_t x[kScanPer];
+    int64_t acc = 0;
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) {
+        x[k] = base + k < n ? v[base + k] : 0;
+        acc += x[k];
+    }
+    const int64_t incl = block_incl_scan(acc, lds);
+    int64_t run = carry_in[blockIdx.x] + incl - acc;
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) {
+        run += x[k];
+        if (base + k < n) v[base + k] = run;
+    }
 }
 
 namespace {
 
 template <typename T, typename IdxT>
 int size_typed(const void *data, int64_t num_vecs, int64_t V, const void *mask, const void *bitval,
-               const void *index, int64_t n, int64_t *sizes, uint64_t *total, hipStream_t stream)
+               const int32_t *clusters, int64_t num_clusters, const void *index, int64_t n, int64_t *sizes,
+               uint64_t *total, hipStream_t stream)
 {
     const DeviceInfo &dev = device_info();
     int64_t grid = (n * kWave + 255) / 256;
     if (grid > (int64_t)dev.num_cus * 8) grid = (int64_t)dev.num_cus * 8;
     if (grid < 1) grid = 1;
     hipLaunchKernelGGL((size_kernel<T, IdxT>), dim3((unsigned)grid), dim3(256), 0, stream, (const T *)data,
-                       num_vecs, V, (const T *)mask, (const T *)bitval, index, n, sizes,
-                       (unsigned long long *)total, index ? index_error_counter() : nullptr);
+                       num_vecs, V, (const T *)mask, (const T *)bitval, clusters, num_clusters, index, n, sizes,
+                       (unsigned long long *)total,
+                       (index || clusters) ? index_error_counter() : nullptr);
     IBP_LAUNCH_CHECK();
     return IBP_OK;
 }
 
 template <typename T>
 int size_dispatch(int index_bytes, const void *data, int64_t num_vecs, int64_t V, const void *mask,
-                  const void *bitval, const void *index, int64_t n, int64_t *sizes, uint64_t *total,
-                  hipStream_t stream)
+                  const void *bitval, const int32_t *clusters, int64_t num_clusters, const void *index, int64_t n,
+                  int64_t *sizes, uint64_t *total, hipStream_t stream)
 {
     if (index && index_bytes == 4)
-        return size_typed<T, int32_t>(data, num_vecs, V, mask, bitval, index, n, sizes, total, stream);
-    return size_typed<T, int64_t>(data, num_vecs, V, mask, bitval, index, n, sizes, total, stream);
+        return size_typed<T, int32_t>(data, num_vecs, V, mask, bitval, clusters, num_clusters, index, n, sizes,
+                                      total, stream);
+    return size_typed<T, int64_t>(data, num_vecs, V, mask, bitval, clusters, num_clusters, index, n, sizes, total,
+                                  stream);
 }
 
-template <typename T, typename IdxT>
-int pack_typed(const void *input, void *output, int64_t num_vecs, int64_t V, const void *mask,
-               const void *bitval, const void *index, int64_t n, int32_t *bitmask, hipStream_t stream)
+int size_any(int elem_bytes, int index_bytes, const void *data, int64_t num_vecs, int64_t V, const void *mask,
+             const void *bitval, const int32_t *clusters, int64_t num_clusters, const void *index, int64_t n,
+             int64_t *sizes, uint64_t *total, hipStream_t s)
+{
+    switch (elem_bytes) {
+    case 1: return size_dispatch<uint8_t>(index_bytes, data, num_vecs, V, mask, bitval, clusters, num_clusters, index, n, sizes, total, s);
+    case 2: return size_dispatch<uint16_t>(index_bytes, data, num_vecs, V, mask, bitval, clusters, num_clusters, index, n, sizes, total, s);
+    case 4: return size_dispatch<uint32_t>(index_bytes, data, num_vecs, V, mask, bitval, clusters, num_clusters, index, n, sizes, total, s);
+    default: return size_dispatch<uint64_t>(index_bytes, data, num_vecs, V, mask, bitval, clusters, num_clusters, index, n, sizes, total, s);
+    }
+}
+
+// In-place inclusive scan of v[n] (device memory) on `stream`.
+int scan_inclusive(int64_t *v, int64_t n, hipStream_t stream)
+{
+    if (n <= 0) return IBP_OK;
+    const int64_t ntiles = (n + kScanTile - 1) / kScanTile;
+    int64_t *sums = nullptr;
+    IBP_HIP(hipMallocAsync((void **)&sums, (size_t)ntiles * sizeof(int64_t), stream));
+    hipLaunchKernelGGL(tile_sum_kernel, dim3((unsigned)ntiles), dim3(256), 0, stream, (const int64_t *)v, n, sums);
+    hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(1024), 0, stream, sums, ntiles);
+    hipLaunchKernelGGL(tile_scan_kernel, dim3((unsigned)ntiles), dim3(256), 0, stream, v, n, (const int64_t *)sums);
+    hipError_t e = hipGetLastError();
+    IBP_HIP(hipFreeAsync(sums, stream));
+    if (e != hipSuccess) return set_error(IBP_ERR_HIP, "scan kernel launch failed: %s", hipGetErrorString(e));
+    return IBP_OK;
+}
+
+template <typename T, typename IdxT, bool CONDENSED>
+int pack_typed(PackParams p, hipStream_t stream)
 {
     const DeviceInfo &dev = device_info();
-    const int64_t slot_bytes = V * (int64_t)sizeof(T);
+    const int64_t buf_bytes = slot_buf_bytes(p.V, (int)sizeof(T));
     const int64_t lds_budget = 64 * 1024;
-    const int64_t ngroups = (n + 31) / 32;
-    unsigned long long *err = index ? index_error_counter() : nullptr;
-    if (slot_bytes <= lds_budget) {
-        int wpb = (int)(lds_budget / slot_bytes);
+    const int64_t ngroups = (p.n + 31) / 32;
+    if (buf_bytes <= lds_budget) {
+        int wpb = (int)(lds_budget / buf_bytes);
         if (wpb > 4) wpb = 4;
         int64_t grid = (ngroups + wpb - 1) / wpb;
         if (grid > (int64_t)dev.num_cus * 16) grid = (int64_t)dev.num_cus * 16;
         if (grid < 1) grid = 1;
-        hipLaunchKernelGGL((pack_kernel<T, IdxT, true>), dim3((unsigned)grid), dim3(wpb * kWave),
-                           (size_t)(wpb * slot_bytes), stream, (const T *)input, (T *)output, num_vecs, V,
-                           (const T *)mask, (const T *)bitval, index, n, (uint32_t *)bitmask, (T *)nullptr, err);
+        p.gbuf = nullptr;
+        hipLaunchKernelGGL((pack_kernel<T, IdxT, true, CONDENSED>), dim3((unsigned)grid), dim3(wpb * kWave),
+                           (size_t)(wpb * buf_bytes), stream, p);
         IBP_LAUNCH_CHECK();
         return IBP_OK;
     }
@@ -211,15 +379,27 @@ int pack_typed(const void *input, void *output, int64_t num_vecs, int64_t V, con
     int64_t nwaves = ngroups < (int64_t)dev.num_cus * 4 ? ngroups : (int64_t)dev.num_cus * 4;
     if (nwaves < 1) nwaves = 1;
     const int64_t grid = (nwaves + 3) / 4;
-    void *gbuf = nullptr;
-    IBP_HIP(hipMallocAsync(&gbuf, (size_t)(grid * 4 * slot_bytes), stream));
-    hipLaunchKernelGGL((pack_kernel<T, IdxT, false>), dim3((unsigned)grid), dim3(256), 0, stream,
-                       (const T *)input, (T *)output, num_vecs, V, (const T *)mask, (const T *)bitval, index, n,
-                       (uint32_t *)bitmask, (T *)gbuf, err);
+    IBP_HIP(hipMallocAsync(&p.gbuf, (size_t)(grid * 4 * buf_bytes), stream));
+    hipLaunchKernelGGL((pack_kernel<T, IdxT, false, CONDENSED>), dim3((unsigned)grid), dim3(256), 0, stream, p);
     hipError_t e = hipGetLastError();
-    IBP_HIP(hipFreeAsync(gbuf, stream));
+    IBP_HIP(hipFreeAsync(p.gbuf, stream));
     if (e != hipSuccess) return set_error(IBP_ERR_HIP, "pack kernel launch failed: %s", hipGetErrorString(e));
     return IBP_OK;
+}
+
+template <typename T>
+int pack_width(const PackParams &p, int index_bytes, hipStream_t s)
+{
+    const bool i32 = p.index && index_bytes == 4;
+    if (p.offsets)
+        return i32 ? pack_typed<T, int32_t, true>(p, s) : pack_typed<T, int64_t, true>(p, s);
+    return i32 ? pack_typed<T, int32_t, false>(p, s) : pack_typed<T, int64_t, false>(p, s);
+}
+
+bool ranges_overlap(const void *a, int64_t an, const void *b, int64_t bn)
+{
+    const uintptr_t a0 = (uintptr_t)a, b0 = (uintptr_t)b;
+    return a0 < b0 + (uintptr_t)bn && b0 < a0 + (uintptr_t)an;
 }
 
 } // namespace
@@ -228,23 +408,109 @@ int pack_typed(const void *input, void *output, int64_t num_vecs, int64_t V, con
 
 using namespace ibp;
 
+#define IBP_CHECK_WIDTH(w)                                                                                  \
+    IBP_CHECK((w) == 1 || (w) == 2 || (w) == 4 || (w) == 8, "Input tensor must be 1, 2, 4 or 8-byte datatype (got %d)", (w))
+
+extern "C" int ibp_get_compress_size_ex(int elem_bytes, const void *data, int64_t num_vecs, int64_t vec_size,
+                                        const void *mask, const void *bitval, const void *index, int index_bytes,
+                                        int64_t n, int64_t *sizes, uint64_t *total, const ibp_layout *layout,
+                                        void *stream)
+{
+    IBP_CHECK_WIDTH(elem_bytes);
+    IBP_CHECK(vec_size > 0 && num_vecs >= 0 && n >= 0, "invalid shape");
+    IBP_CHECK(index == nullptr || index_bytes == 4 || index_bytes == 8, "index must be int32 or int64");
+    IBP_CHECK(index != nullptr || n <= num_vecs, "without an index n must not exceed num_vecs");
+    IBP_CHECK(layout == nullptr || layout->offsets == nullptr, "sizes take no condensed offsets");
+    const int32_t *clusters = layout ? layout->clusters : nullptr;
+    const int64_t k = layout && clusters ? layout->num_clusters : 1;
+    IBP_CHECK(k >= 1, "num_clusters must be >= 1");
+    if (n == 0) return IBP_OK;
+    IBP_CHECK(data && mask && bitval && sizes, "null pointer argument");
+    return size_any(elem_bytes, index_bytes, data, num_vecs, vec_size, mask, bitval, clusters, k, index, n, sizes,
+                    total, as_stream(stream));
+}
+
 extern "C" int ibp_get_compress_size(int elem_bytes, const void *data, int64_t num_vecs, int64_t vec_size,
                                      const void *mask, const void *bitval, const void *index, int index_bytes,
                                      int64_t n, int64_t *sizes, uint64_t *total, void *stream)
 {
-    IBP_CHECK(elem_bytes == 1 || elem_bytes == 2 || elem_bytes == 4 || elem_bytes == 8,
-              "Input tensor must be 1, 2, 4, or 8-byte datatype (got %d)", elem_bytes);
+    return ibp_get_compress_size_ex(elem_bytes, data, num_vecs, vec_size, mask, bitval, index, index_bytes, n,
+                                    sizes, total, nullptr, stream);
+}
+
+extern "C" int ibp_condensed_offsets(int elem_bytes, const void *data, int64_t num_vecs, int64_t vec_size,
+                                     const void *mask, const void *bitval, const void *index, int index_bytes,
+                                     int64_t n, int64_t *offsets, const ibp_layout *layout, void *stream)
+{
+    IBP_CHECK_WIDTH(elem_bytes);
     IBP_CHECK(vec_size > 0 && num_vecs >= 0 && n >= 0, "invalid shape");
     IBP_CHECK(index == nullptr || index_bytes == 4 || index_bytes == 8, "index must be int32 or int64");
     IBP_CHECK(index != nullptr || n <= num_vecs, "without an index n must not exceed num_vecs");
+    IBP_CHECK(layout == nullptr || layout->offsets == nullptr, "layout->offsets is the output here: pass it as offsets");
+    const int32_t *clusters = layout ? layout->clusters : nullptr;
+    const int64_t k = layout && clusters ? layout->num_clusters : 1;
+    IBP_CHECK(k >= 1, "num_clusters must be >= 1");
     if (n == 0) return IBP_OK;
-    IBP_CHECK(data && mask && bitval && sizes, "null pointer argument");
+    IBP_CHECK(data && mask && bitval && offsets, "null pointer argument");
+    hipStream_t s = as_stream(stream);
+    // Sizes are written and scanned in device memory; host (pinned) offsets
+    // receive one copy at the end.
+    const bool host = pointer_is_host(offsets);
+    int64_t *dev = offsets;
+    if (host) IBP_HIP(hipMallocAsync((void **)&dev, (size_t)n * sizeof(int64_t), s));
+    int st = size_any(elem_bytes, index_bytes, data, num_vecs, vec_size, mask, bitval, clusters, k, index, n, dev,
+                      nullptr, s);
+    if (st == IBP_OK) st = scan_inclusive(dev, n, s);
+    if (host) {
+        if (st == IBP_OK) IBP_HIP(hipMemcpyAsync(offsets, dev, (size_t)n * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        IBP_HIP(hipFreeAsync(dev, s));
+    }
+    return st;
+}
+
+extern "C" int ibp_compress_ex(int elem_bytes, const void *input, void *output, int64_t num_vecs, int64_t vec_size,
+                               const void *mask, const void *bitval, const void *index, int index_bytes, int64_t n,
+                               int32_t *bitmask, const ibp_layout *layout, void *stream)
+{
+    IBP_CHECK_WIDTH(elem_bytes);
+    IBP_CHECK(vec_size > 0 && num_vecs >= 0 && n >= 0, "invalid shape");
+    IBP_CHECK(index == nullptr || index_bytes == 4 || index_bytes == 8, "index must be int32 or int64");
+    IBP_CHECK(index != nullptr || n <= num_vecs, "without an index n must not exceed num_vecs");
+    const bool condensed = layout && layout->offsets;
+    const int32_t *clusters = layout ? layout->clusters : nullptr;
+    const int64_t k = clusters ? layout->num_clusters : 1;
+    IBP_CHECK(k >= 1, "num_clusters must be >= 1");
+    IBP_CHECK(condensed || index == nullptr || input != output,
+              "an index with in-place output aliases rows (unsupported)");
+    if (condensed) {
+        IBP_CHECK(layout->stream_bytes >= 0, "stream_bytes must be >= 0");
+        IBP_CHECK(!ranges_overlap(output, layout->stream_bytes, input, num_vecs * vec_size * elem_bytes),
+                  "the condensed stream must not overlap the input");
+    }
+    if (n == 0) return IBP_OK;
+    IBP_CHECK(input && output && mask && bitval && (bitmask || condensed), "null pointer argument");
+    PackParams p;
+    p.in = input;
+    p.out = output;
+    p.num_vecs = num_vecs;
+    p.V = vec_size;
+    p.mask = mask;
+    p.bitval = bitval;
+    p.clusters = clusters;
+    p.num_clusters = k;
+    p.offsets = condensed ? layout->offsets : nullptr;
+    p.stream_bytes = condensed ? layout->stream_bytes : 0;
+    p.index = index;
+    p.n = n;
+    p.bitmask = (uint32_t *)bitmask;
+    p.gbuf = nullptr;
+    p.err = (index || clusters || condensed) ? index_error_counter() : nullptr;
     hipStream_t s = as_stream(stream);
     switch (elem_bytes) {
-    case 1: return size_dispatch<uint8_t>(index_bytes, data, num_vecs, vec_size, mask, bitval, index, n, sizes, total, s);
-    case 2: return size_dispatch<uint16_t>(index_bytes, data, num_vecs, vec_size, mask, bitval, index, n, sizes, total, s);
-    case 4: return size_dispatch<uint32_t>(index_bytes, data, num_vecs, vec_size, mask, bitval, index, n, sizes, total, s);
-    default: return size_dispatch<uint64_t>(index_bytes, data, num_vecs, vec_size, mask, bitval, index, n, sizes, total, s);
+    case 1: return pack_width<uint8_t>(p, index_bytes, s);
+    case 2: return pack_width<uint16_t>(p, index_bytes, s);
+    case 4: return pack_width<uint32_t>(p, index_bytes, s);
+    default: return pack_width<uint64_t>(p, index_bytes, s);
     }
 }
 
@@ -252,18 +518,6 @@ extern "C" int ibp_compress(int elem_bytes, const void *input, void *output, int
                             const void *mask, const void *bitval, const void *index, int index_bytes, int64_t n,
                             int32_t *bitmask, void *stream)
 {
-    IBP_CHECK(elem_bytes == 4 || elem_bytes == 8, "Input tensor must be 4-byte or 8-byte datatype (got %d)", elem_bytes);
-    IBP_CHECK(vec_size > 0 && num_vecs >= 0 && n >= 0, "invalid shape");
-    IBP_CHECK(index == nullptr || index_bytes == 4 || index_bytes == 8, "index must be int32 or int64");
-    IBP_CHECK(index != nullptr || n <= num_vecs, "without an index n must not exceed num_vecs");
-    IBP_CHECK(index == nullptr || input != output, "an index with in-place output aliases rows (unsupported)");
-    if (n == 0) return IBP_OK;
-    IBP_CHECK(input && output && mask && bitval && bitmask, "null pointer argument");
-    hipStream_t s = as_stream(stream);
-    const bool i32 = index && index_bytes == 4;
-    if (elem_bytes == 8)
-        return i32 ? pack_typed<uint64_t, int32_t>(input, output, num_vecs, vec_size, mask, bitval, index, n, bitmask, s)
-                   : pack_typed<uint64_t, int64_t>(input, output, num_vecs, vec_size, mask, bitval, index, n, bitmask, s);
-    return i32 ? pack_typed<uint32_t, int32_t>(input, output, num_vecs, vec_size, mask, bitval, index, n, bitmask, s)
-               : pack_typed<uint32_t, int64_t>(input, output, num_vecs, vec_size, mask, bitval, index, n, bitmask, s);
+    return ibp_compress_ex(elem_bytes, input, output, num_vecs, vec_size, mask, bitval, index, index_bytes, n,
+                           bitmask, nullptr, stream);
 }

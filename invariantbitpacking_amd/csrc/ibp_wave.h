@@ -37,6 +37,8 @@ __device__ __forceinline__ uint64_t readlane(uint64_t x, int lane)
     return ((uint64_t)hi << 32) | lo;
 }
 __device__ __forceinline__ int64_t readlane(int64_t x, int lane) { return (int64_t)readlane((uint64_t)x, lane); }
+__device__ __forceinline__ uint8_t readlane(uint8_t x, int lane) { return (uint8_t)readlane((uint32_t)x, lane); }
+__device__ __forceinline__ uint16_t readlane(uint16_t x, int lane) { return (uint16_t)readlane((uint32_t)x, lane); }
 
 __device__ __forceinline__ int readfirstlane(int x) { return __builtin_amdgcn_readfirstlane(x); }
 __device__ __forceinline__ int64_t readfirstlane(int64_t x)
@@ -59,6 +61,8 @@ __device__ __forceinline__ uint32_t shfl(uint32_t x, int src)
 {
     return (uint32_t)__builtin_amdgcn_ds_bpermute(src << 2, (int)x);
 }
+__device__ __forceinline__ uint8_t shfl(uint8_t x, int src) { return (uint8_t)shfl((uint32_t)x, src); }
+__device__ __forceinline__ uint16_t shfl(uint16_t x, int src) { return (uint16_t)shfl((uint32_t)x, src); }
 __device__ __forceinline__ uint64_t shfl(uint64_t x, int src)
 {
     const uint32_t lo = (uint32_t)__builtin_amdgcn_ds_bpermute(src << 2, (int)(uint32_t)x);

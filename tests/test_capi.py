@@ -45,7 +45,7 @@ def test_selftest_bitops_host(capi):
 def test_invalid_arguments_are_status_codes(capi):
     L = capi.lib()
     st = L.ibp_decompress_fetch(3, None, None, 10, 4, None, None, None, None, 0, 10, 0, 0, 0, -1, 0, None)
-    assert st == 1 and "4-byte or 8-byte" in capi.last_error()
+    assert st == 1 and "1, 2, 4 or 8-byte" in capi.last_error()
     st = L.ibp_decompress_fetch(8, None, None, 10, 4, None, None, None, None, 0, 10, 0, 0, 0, 9, 0, None)
     assert st == 1 and "impl" in capi.last_error()
     st = L.ibp_decompress_fetch(8, None, None, 10, 4, None, None, None, None, 0, 11, 0, 0, 0, -1, 0, None)
@@ -55,7 +55,7 @@ def test_invalid_arguments_are_status_codes(capi):
     buf = ctypes.create_string_buffer(64)
     st = L.ibp_compress(8, buf, buf, 2, 4, buf, buf, buf, 8, 2, buf, None)
     assert st == 1 and "alias" in capi.last_error()
-    st = L.ibp_compress(2, buf, buf, 2, 4, buf, buf, None, 0, 2, buf, None)
+    st = L.ibp_compress(3, buf, buf, 2, 4, buf, buf, None, 0, 2, buf, None)
     assert st == 1
     st = L.ibp_preprocess(3, buf, 2, 4, 0.9, buf, buf, None, None, 0, None)
     assert st == 1 and "1, 2, 4, or 8-byte" in capi.last_error()
@@ -63,6 +63,22 @@ def test_invalid_arguments_are_status_codes(capi):
     assert st == 1 and "num_clusters" in capi.last_error()
     st = L.ibp_get_compress_size(8, buf, 2, 4, buf, buf, None, 0, 3, buf, None, None)
     assert st == 1
+    # layouts: transfer baselines take none; the condensed stream must not
+    # overlap its input; condensed offsets are the output of ibp_condensed_offsets
+    cl = (ctypes.c_int32 * 2)(0, 0)
+    lay = capi.Layout(None, 0, ctypes.addressof(cl), 1)
+    st = L.ibp_decompress_fetch_ex(8, buf, buf, 2, 4, buf, buf, buf, None, 0, 2, 0, 0, 0, 2, 0, ctypes.byref(lay),
+                                   None)
+    assert st == 1 and "transfer baseline" in capi.last_error()
+    offs = (ctypes.c_int64 * 2)(32, 64)
+    lay = capi.Layout(ctypes.addressof(offs), 64, None, 1)
+    st = L.ibp_compress_ex(8, buf, buf, 2, 4, buf, buf, None, 0, 2, None, ctypes.byref(lay), None)
+    assert st == 1 and "overlap" in capi.last_error()
+    st = L.ibp_condensed_offsets(8, buf, 2, 4, buf, buf, None, 0, 2, offs, ctypes.byref(lay), None)
+    assert st == 1 and "output" in capi.last_error()
+    lay = capi.Layout(None, 0, ctypes.addressof(cl), 0)
+    st = L.ibp_get_compress_size_ex(8, buf, 2, 4, buf, buf, None, 0, 2, buf, None, ctypes.byref(lay), None)
+    assert st == 1 and "num_clusters" in capi.last_error()
     plan = ctypes.c_void_p()
     assert L.ibp_plan_create(None, 0, ctypes.byref(plan)) == 1
     cfg = capi.ServerConfig(16, 1, 1, 1)
@@ -91,8 +107,12 @@ def test_python_api_rejects_unreachable_memory():
         ibp.compress_inplace(x, m, m)
     with pytest.raises(RuntimeError, match="accessible by CUDA device"):
         ibp.get_compress_size(x, m, m)
-    with pytest.raises(NotImplementedError):
+    with pytest.raises(RuntimeError, match="accessible by CUDA device"):
         ibp.compress(x, m, m)
+    with pytest.raises(RuntimeError, match="accessible by CUDA device"):
+        ibp.compress_condensed(x, m, m)
+    with pytest.raises(RuntimeError, match="accessible by CUDA device"):
+        ibp.decompress(x.view(-1), torch.zeros(4, dtype=torch.int64), m, m, torch.device("cuda"), 8)
 
 
 def test_reference_api_surface():
@@ -110,9 +130,40 @@ def test_reference_api_surface():
         "compress": ["dataset", "mask", "bitval", "index_arr"],
         "decompress_fetch": ["comp_dataset", "mask", "bitval", "bitmask", "device", "comp_len", "index_arr",
                              "nblks", "nthds", "imp", "output_tensor"],
+        # the extension module's further bindings (src/ibp_api.cu:131,133)
+        "compress_condensed": ["dataset", "mask", "bitval", "index_arr"],
+        "read_shared": ["filename", "shape", "dtype"],
     }
     for name, params in want.items():
         sig = inspect.signature(getattr(ibp, name))
         positional = [p.name for p in sig.parameters.values()
                       if p.kind in (p.POSITIONAL_ONLY, p.POSITIONAL_OR_KEYWORD)]
         assert positional == params, name
+
+
+def test_read_shared_loads_file_into_shared_memory(tmp_path):
+    """read_shared (src/ibp_api.cu:52-122): the file's bytes in a shared-memory
+    tensor of the given shape and dtype; no GPU involved."""
+    import numpy as np
+
+    import invariantbitpacking_amd as ibp
+    rng = np.random.default_rng(0)
+    data = rng.integers(-2**62, 2**62, size=(1000, 37), dtype=np.int64)
+    f = tmp_path / "rows.bin"
+    data.tofile(f)
+    t = ibp.read_shared(str(f), [1000, 37], torch.int64)
+    assert t.is_shared() and t.shape == (1000, 37) and t.dtype == torch.int64
+    assert np.array_equal(t.numpy(), data)
+    # several reader threads, pieces larger than one thread's share
+    big = rng.integers(0, 255, size=(150 << 20) // 4, dtype=np.int32)
+    g = tmp_path / "big.bin"
+    big.tofile(g)
+    t2 = ibp.read_shared(str(g), [big.size], torch.int32, threads=5)
+    assert np.array_equal(t2.numpy(), big)
+    # a shorter file leaves the rest zero; a longer one is an error
+    t3 = ibp.read_shared(str(f), [1100, 37], torch.int64)
+    assert np.array_equal(t3[:1000].numpy(), data) and not t3[1000:].any()
+    with pytest.raises(RuntimeError, match="more than"):
+        ibp.read_shared(str(f), [999, 37], torch.int64)
+    with pytest.raises(RuntimeError, match="cannot open"):
+        ibp.read_shared(str(tmp_path / "missing.bin"), [4], torch.int64)
