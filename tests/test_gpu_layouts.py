@@ -90,7 +90,7 @@ def test_condensed_with_index_and_output_tensor(ibp, oracle):
         stream, off = ibp.compress_condensed(xd, mask, bitval, idx)
         torch.cuda.synchronize()
         assert np.array_equal(off.cpu().numpy(), wo)
-        assert np.array_equal(stream.cpu().numpy().view(np.uint64), ws)
+        assert np.array_equal(stream.cpu().numpy().view(ws.dtype), ws)
     out = torch.full((777, v), 3, dtype=torch.int64, device="cuda")
     r = ibp.decompress(stream, off, mask, bitval, torch.device("cuda"), v, None, out)
     torch.cuda.synchronize()
@@ -165,7 +165,7 @@ def test_clusters_index_uses_source_rows_and_bad_ids_are_counted(ibp, oracle):
     ws, wo = oracle.compress_condensed(x, masks_np, bitvals_np, index=sel, clusters=c)
     stream, off = ibp.compress_condensed(_dev(x), masks, bitvals, torch.from_numpy(sel).cuda(), clusters=cl)
     torch.cuda.synchronize()
-    assert np.array_equal(off.cpu().numpy(), wo) and np.array_equal(stream.cpu().numpy().view(np.uint64), ws)
+    assert np.array_equal(off.cpu().numpy(), wo) and np.array_equal(stream.cpu().numpy().view(ws.dtype), ws)
     stored_cl = cl[torch.from_numpy(sel).cuda()]
     back = ibp.decompress(stream, off, masks, bitvals, torch.device("cuda"), v, clusters=stored_cl)
     torch.cuda.synchronize()
@@ -176,7 +176,7 @@ def test_clusters_index_uses_source_rows_and_bad_ids_are_counted(ibp, oracle):
     bm = ibp.compress_inplace(slots, masks, bitvals, torch.from_numpy(sel).cuda(), clusters=cl)
     want, wbm = oracle.compress(x, masks_np, bitvals_np, index=sel, clusters=c)
     assert np.array_equal(bm.cpu().numpy(), wbm)
-    assert np.array_equal(slots[:300].cpu().numpy().view(np.uint64), want)
+    assert np.array_equal(slots[:300].cpu().numpy().view(want.dtype), want)
     assert torch.equal(slots[300:].cpu(), torch.from_numpy(x[300:]))
     # an invalid cluster id gives a zero row and is counted
     bad = stored_cl.clone()
