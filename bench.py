@@ -101,6 +101,40 @@ def cpu_baseline(comp_np, om, ob, obm, idx_list, budget_s=10.0):
                       "core (%.1f s)" % (batches, len(idx_list[0]), t_total)}
 
 
+def host_info():
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
+    except OSError:
+        pass
+    return {"cpus": len(os.sched_getaffinity(0)), "model": model}
+
+
+def cpu_compress_baseline(x_np, rows=100000):
+    """Host-side compression by the C oracle on the first `rows` rows:
+    preprocess (threshold sweep) on one core, pack on one core and on every
+    core available to the process (OpenMP)."""
+    from oracle import oracle
+    sample = np.ascontiguousarray(x_np[:rows])
+    nbytes = sample.nbytes
+    t0 = time.perf_counter()
+    m, b, _, _ = oracle.preprocess(sample, None)
+    t_pre = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    oracle.compress(sample, m, b)
+    t_1 = time.perf_counter() - t0
+    threads = len(os.sched_getaffinity(0))
+    oracle.compress_mt(sample, m, b, threads)  # warm the thread pool
+    t0 = time.perf_counter()
+    oracle.compress_mt(sample, m, b, threads)
+    t_mt = time.perf_counter() - t0
+    return {"sample": "%d rows x %d int64 (%.1f MB)" % (sample.shape[0], sample.shape[1], nbytes / 1e6),
+            "preprocess_1core_gbps": round(nbytes / t_pre / 1e9, 3),
+            "compress_1core_gbps": round(nbytes / t_1 / 1e9, 3),
+            "compress_all_cores_gbps": round(nbytes / t_mt / 1e9, 3), "threads": threads}
+
+
 def algorithmic_hbm_bytes(sizes_sel, n, v, w):
     """Per launch: compressed bytes read + decompressed bytes written + int64
     index + bitmask words (ceil(n/32)*4)."""
@@ -159,10 +193,23 @@ def run_gnn_minibatch(args):
     num_rows, v = x.shape
     w = 8
     log("data %s generated in %.1fs" % (tuple(x.shape), time.perf_counter() - t0))
+    warm = x[:4096].clone()  # first calls load the kernels: keep that out of the timings
+    wm, wb = ibp.preprocess(warm, None)
+    ibp.compress_inplace(warm, wm, wb)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
     mask, bitval = ibp.preprocess(x, None)
+    torch.cuda.synchronize()
+    t_pre = time.perf_counter() - t0
     sizes = ibp.get_compress_size(x, mask, bitval)
     comp = x.clone()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
     bitmask = ibp.compress_inplace(comp, mask, bitval)
+    t_pack = time.perf_counter() - t0
+    gpu_compress = {"preprocess_gbps": round(x.numel() * w / t_pre / 1e9, 1),
+                    "compress_inplace_gbps": round(x.numel() * w / t_pack / 1e9, 1),
+                    "what": "whole table in HBM, wall clock around each synchronised call"}
     comp_len = ibp.get_single_compress_len(x, mask, bitval)
     torch.cuda.synchronize()
     packed = int(torch.count_nonzero((sizes < v * w)))
@@ -207,7 +254,7 @@ def run_gnn_minibatch(args):
     pcie = None
     if not args.no_pcie:
         pcie = run_pcie(ibp, comp, mask, bitval, bitmask, comp_len, idx, x, out, sel, max(3, args.steps // 2))
-    cpu = None
+    cpu = cpu_compress = None
     if not args.no_cpu:
         comp_np = comp.cpu().numpy()
         om = mask.cpu().numpy().view(np.uint64)
@@ -215,6 +262,8 @@ def run_gnn_minibatch(args):
         obm = bitmask.cpu().numpy()
         cpu = cpu_baseline(comp_np, om, ob, obm, [i.numpy() for i in idx_cpu], budget_s=args.cpu_seconds)
         log("cpu baseline: %s" % cpu)
+        cpu_compress = cpu_compress_baseline(x[:100000].cpu().numpy())
+        log("cpu compression: %s" % cpu_compress)
 
     return {
         "metric": METRIC, "value": round(value, 2), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
@@ -232,6 +281,8 @@ def run_gnn_minibatch(args):
                        "what": "isolated call: decompress_fetch issued -> rows ready (host wall clock)"},
         "batched": batched,
         "pcie_resident": pcie,
+        "compression": {"gpu": gpu_compress, "cpu": cpu_compress},
+        "host": host_info(),
     }
 
 
@@ -285,6 +336,61 @@ def run_dlrm(args):
            "batched_vs_plain": round(p_ms / b_ms, 3), "batched_hbm_alg_gbps": round(alg / (b_ms / 1e3) / 1e9, 1),
            "plan": json.loads(bf.describe()), "bitexact": exact and b_exact}
     bf.close()
+    res["condensed"] = run_dlrm_condensed(ibp, args, sets, idx, outs, raw_bytes)
+    return res
+
+
+def run_dlrm_condensed(ibp, args, sets, idx, outs, raw_bytes):
+    """The same tables as condensed streams (compress_condensed): bytes kept,
+    the fetch from a device-resident stream, and -- the deployment the stream
+    exists for -- the fetch from pinned host memory over PCIe, next to the
+    uncompressed pinned row gather (the reference's UVA baseline,
+    tests/decompression_thput2.py:40-49)."""
+    streams = []
+    for s in sets:
+        st, off = ibp.compress_condensed(s["x"], s["mask"], s["bitval"])
+        streams.append((st, off))
+    torch.cuda.synchronize()
+    stream_bytes = sum(st.numel() * 8 for st, _ in streams)
+    dec = len(sets) * args.batch * 512
+
+    def fetch(src):
+        for (st, off), s, i, o in zip(src, sets, idx, outs):
+            ibp.decompress(st, off, s["mask"], s["bitval"], torch.device("cuda"), 64, i, o, s["comp_len"])
+    d_ms, _ = timed_region(lambda: fetch(streams), args.steps, args.warmup)
+    d_exact = all(bool(torch.equal(o, s["x"][i])) for s, i, o in zip(sets, idx, outs))
+    res = {"stream_bytes": stream_bytes, "raw_bytes": raw_bytes, "host_footprint_ratio": round(raw_bytes / stream_bytes, 3),
+           "device": {"ms_per_batch": round(d_ms, 4), "gbps": round(dec / (d_ms / 1e3) / 1e9, 1), "bitexact": d_exact}}
+    log("dlrm condensed (device): %s" % res)
+    if args.no_pcie:
+        return res
+    pinned = [(st.cpu().pin_memory(), off) for st, off in streams]
+    del streams
+    torch.cuda.synchronize()
+    steps = max(3, args.steps // 8)
+    p_ms, _ = timed_region(lambda: fetch(pinned), steps, 1)
+    p_exact = all(bool(torch.equal(o, s["x"][i])) for s, i, o in zip(sets, idx, outs))
+    lens = [off.diff(prepend=torch.zeros(1, dtype=torch.int64, device=off.device)) for _, off in pinned]
+    pcie = sum(int(l[i].sum()) for l, i in zip(lens, idx))
+    res["pinned"] = {"ms_per_batch": round(p_ms, 3), "decompressed_gbps": round(dec / (p_ms / 1e3) / 1e9, 2),
+                     "compressed_pcie_gbps": round(pcie / (p_ms / 1e3) / 1e9, 2), "pcie_bytes_per_batch": pcie,
+                     "bitexact": p_exact}
+    del pinned
+    # uncompressed rows in pinned memory, gathered by the GPU (impl 2: raw rows, no unpack)
+    raw = [s["x"].cpu().pin_memory() for s in sets]
+    zero = [torch.zeros_like(s["bitmask"]) for s in sets]
+    torch.cuda.synchronize()
+
+    def gather():
+        for r, z, s, i, o in zip(raw, zero, sets, idx, outs):
+            ibp.decompress_fetch(r, s["mask"], s["bitval"], z, torch.device("cuda"), 64, i, None, None, 2, o)
+    g_ms, _ = timed_region(gather, steps, 1)
+    g_exact = all(bool(torch.equal(o, s["x"][i])) for s, i, o in zip(sets, idx, outs))
+    res["raw_pinned_gather"] = {"ms_per_batch": round(g_ms, 3), "gbps": round(dec / (g_ms / 1e3) / 1e9, 2),
+                                "bitexact": g_exact}
+    res["pinned_vs_raw_gather"] = round(g_ms / p_ms, 3)
+    log("dlrm condensed (pinned): %s" % {k: res[k] for k in ("pinned", "raw_pinned_gather", "pinned_vs_raw_gather")})
+    del raw
     return res
 
 
@@ -371,18 +477,25 @@ def run_serve(args):
             "server_stats": {k: (round(v, 2) if isinstance(v, float) else v) for k, v in st.items()}}
 
 
+def dec_bytes(nbytes, ms):
+    """GB/s of nbytes moved in ms milliseconds."""
+    return nbytes / (ms / 1e3) / 1e9
+
+
 def run_llm(args):
     """LLM weight offload: fp16 N(0, 0.02) blocks [4096 x 4096] x4 and
     [4096 x 11008] x3 per decoder layer, viewed as int64 [4096 x 1024] /
     [4096 x 2752]; preprocess_kmeans(16) per block; compress_inplace in pinned
     host memory; layer-by-layer decompress_fetch (the kernel reads host memory
-    over PCIe) while a bf16 GEMM loop runs on a second stream."""
+    over PCIe) while a bf16 GEMM loop runs on a second stream.  Coded twice:
+    with mask row 0 for every row (what the reference's compress_inplace
+    applies) and with each row's own cluster mask (clusters=)."""
     import invariantbitpacking_amd as ibp
     from invariantbitpacking_amd import workloads
     layers = workloads.llm_weight_blocks(layers=args.layers)
     blocks = []
     t_km = t_pack = 0.0
-    raw = comp_b = 0
+    raw = comp_b = comp_c = 0
     for li, layer in enumerate(layers):
         for w in layer:
             xd = w.cuda()
@@ -400,9 +513,13 @@ def run_llm(args):
             t_pack += time.perf_counter() - t0
             raw += w.numel() * 8
             comp_b += int(sizes.sum())
+            cl = clusters.cuda()
+            comp_c += int(ibp.get_compress_size(xd, mask, bitval, clusters=cl).sum())
+            host_c = w.clone().pin_memory()
+            bm_c = ibp.compress_inplace(host_c, mask, bitval, clusters=cl)
             blocks.append(dict(x=w, host=host, mask=mask, bitval=bitval, bitmask=bm,
                                comp_len=ibp.get_single_compress_len(xd, mask, bitval),
-                               out=torch.empty_like(xd), dev=None))
+                               out=torch.empty_like(xd), dev=None, host_c=host_c, bitmask_c=bm_c, clusters=cl))
             del xd
     log("llm: %d layers, %d blocks, ratio %.4f, kmeans %.2fs, pack %.2fs" % (len(layers), len(blocks),
                                                                             raw / comp_b, t_km, t_pack))
@@ -410,12 +527,20 @@ def run_llm(args):
 
     def fetch_layer(li, src="host"):
         for b in blocks[li * per_layer:(li + 1) * per_layer]:
-            ibp.decompress_fetch(b[src], b["mask"], b["bitval"], b["bitmask"], torch.device("cuda"), b["comp_len"],
-                                 None, None, None, None, b["out"])
+            if src == "host_c":
+                ibp.decompress_fetch(b[src], b["mask"], b["bitval"], b["bitmask_c"], torch.device("cuda"),
+                                     b["comp_len"], None, None, None, None, b["out"], clusters=b["clusters"])
+            else:
+                ibp.decompress_fetch(b[src], b["mask"], b["bitval"], b["bitmask"], torch.device("cuda"),
+                                     b["comp_len"], None, None, None, None, b["out"])
 
     def fetch_all(src="host"):
         for li in range(len(layers)):
             fetch_layer(li, src)
+    c_ms, _ = timed_region(lambda: fetch_all("host_c"), 3, 1)
+    c_exact = all(bool(torch.equal(b["out"].cpu(), b["x"])) for b in blocks)
+    for b in blocks:
+        del b["host_c"]
     f_ms, _ = timed_region(fetch_all, 3, 1)
     exact = all(bool(torch.equal(b["out"].cpu(), b["x"])) for b in blocks)
     # device-resident fetch of the same blocks (wide-row kernel against HBM)
@@ -431,6 +556,11 @@ def run_llm(args):
     def gemms(k):
         for _ in range(k):
             torch.matmul(a, bb)
+    # the first GEMM on a new stream sets up its BLAS workspace (milliseconds,
+    # with a device synchronisation): keep that out of the overlap timing
+    with torch.cuda.stream(side):
+        gemms(1)
+    torch.cuda.synchronize()
     g_ms, _ = timed_region(lambda: gemms(4), 3, 1)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -443,6 +573,10 @@ def run_llm(args):
     dec = raw
     return {"workload": "llm offload: %d layers x (4 x [4096x4096] + 3 x [4096x11008]) fp16 as int64 rows of "
                         "1024/2752, kmeans(16) masks (row 0 used by the codec)" % len(layers),
+            "per_cluster": {"compression_ratio": round(raw / comp_c, 4),
+                            "fetch_pinned_ms_per_layer": round(c_ms / len(layers), 3),
+                            "fetch_pinned_decompressed_gbps": round(dec_bytes(raw, c_ms), 2),
+                            "fetch_pinned_pcie_gbps": round(dec_bytes(comp_c, c_ms), 2), "bitexact": c_exact},
             "compression_ratio": round(raw / comp_b, 4), "kmeans_s_per_block": round(t_km / len(blocks), 3),
             "compress_inplace_pinned_gbps": round(raw / t_pack / 1e9, 2),
             "fetch_pinned_ms_per_layer": round(f_ms / len(layers), 3),

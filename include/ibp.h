@@ -159,7 +159,9 @@ int ibp_compress_ex(int elem_bytes, const void *input, void *output, int64_t num
  * elements (<= 0 means vec_size), a performance hint only.  impl: -1 auto,
  * 0/1/5 unpack (identical output), 2 raw gather without unpack, 3/4 the
  * reference's transfer-only baselines (wrong output by design).  nblks/nthreads
- * <= 0 pick defaults.  Out-of-range indices produce zero rows and are counted
+ * <= 0 pick defaults.  comp_loc (ibp_loc) says where comp lives; AUTO asks the
+ * runtime.  Host-resident rows of <= 64 elements default to half a block per
+ * CU: enough to saturate PCIe, leaving CUs to other streams' kernels.  Out-of-range indices produce zero rows and are counted
  * (ibp_index_errors).  No allocation, no host synchronisation: safe under
  * stream capture. */
 int ibp_decompress_fetch(int elem_bytes, void *out, const void *comp, int64_t num_vecs,

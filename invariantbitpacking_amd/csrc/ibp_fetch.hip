@@ -148,7 +148,7 @@ int launch_width(const FetchParams &p, int layout, int index_bytes, int impl, in
 } // namespace
 
 int fetch_launch(const FetchParams &p, int elem_bytes, int index_bytes, int impl, int comp_len, int nblks,
-                 int nthreads, hipStream_t stream, int layout)
+                 int nthreads, hipStream_t stream, int layout, bool host_resident)
 {
     const DeviceInfo &dev = device_info();
     // Blocks are at most 256 threads (four waves); larger requests are capped.
@@ -156,7 +156,11 @@ int fetch_launch(const FetchParams &p, int elem_bytes, int index_bytes, int impl
     const int rows = g_rows_in_flight.load();
     int64_t waves_needed = p.vec_size <= kWave ? (p.n + rows - 1) / rows : p.n;
     int64_t grid = nblks > 0 ? nblks : (waves_needed * kWave + block - 1) / block;
-    const int64_t cap = (int64_t)dev.num_cus * g_blocks_per_cu.load();
+    int64_t cap = (int64_t)dev.num_cus * g_blocks_per_cu.load();
+    // Small rows read over PCIe: half a block per CU keeps enough rows in
+    // flight to saturate the link (measured: 32..128 blocks run at the full
+    // rate on MI355X) and leaves the other CUs to kernels on other streams.
+    if (host_resident && p.vec_size <= kWave) cap = dev.num_cus / 2 > 0 ? dev.num_cus / 2 : 1;
     if (nblks <= 0 && grid > cap) grid = cap;
     if (grid < 1) grid = 1;
     switch (elem_bytes) {
@@ -188,7 +192,6 @@ extern "C" int ibp_decompress_fetch_ex(int elem_bytes, void *out, const void *co
                                        int comp_len, int nblks, int nthreads, int impl, int comp_loc,
                                        const ibp_layout *layout, void *stream)
 {
-    (void)comp_loc;
     IBP_CHECK(elem_bytes == 1 || elem_bytes == 2 || elem_bytes == 4 || elem_bytes == 8,
               "Input tensor must be 1, 2, 4 or 8-byte datatype (got %d)", elem_bytes);
     IBP_CHECK(vec_size > 0 && num_vecs >= 0 && n >= 0, "invalid shape num_vecs=%lld vec_size=%lld n=%lld",
@@ -231,7 +234,8 @@ extern "C" int ibp_decompress_fetch_ex(int elem_bytes, void *out, const void *co
         p.num_clusters = layout->num_clusters;
     }
     const int cl = comp_len > 0 ? comp_len : (int)vec_size;
-    return fetch_launch(p, elem_bytes, index_bytes, impl, cl, nblks, nthreads, as_stream(stream), lay);
+    const bool host = comp_loc == IBP_LOC_HOST || (comp_loc == IBP_LOC_AUTO && pointer_is_host(comp));
+    return fetch_launch(p, elem_bytes, index_bytes, impl, cl, nblks, nthreads, as_stream(stream), lay, host);
 }
 
 extern "C" int ibp_decompress_fetch(int elem_bytes, void *out, const void *comp, int64_t num_vecs,

@@ -356,6 +356,23 @@ void oracle_compress_clustered(int w, const void *input, void *output, int64_t n
     free(slot);
 }
 
+void oracle_compress_mt(int w, const void *input, void *output, int64_t num_vecs, int64_t vec_size,
+                        const void *masks, const void *bitvals, const int32_t *clusters, const int64_t *index,
+                        int64_t n, int32_t *bitmask, int threads)
+{
+    const int64_t vec_bytes = vec_size * w;
+    const int64_t groups = (n + 31) / 32;
+    /* 32 rows per task: a task owns its bitmask word. */
+#pragma omp parallel for schedule(dynamic, 16) num_threads(threads)
+    for (int64_t g = 0; g < groups; ++g) {
+        const int64_t lo = g * 32, cnt = (lo + 32 < n ? lo + 32 : n) - lo;
+        const uint8_t *in = (const uint8_t *)input + (index ? 0 : lo * vec_bytes);
+        const int32_t *cl = clusters && !index ? clusters + lo : clusters;
+        oracle_compress_clustered(w, in, (uint8_t *)output + lo * vec_bytes, num_vecs, vec_size, masks, bitvals,
+                                  cl, index ? index + lo : NULL, cnt, bitmask + g);
+    }
+}
+
 void oracle_compress(int w, const void *input, void *output, int64_t num_vecs, int64_t vec_size,
                      const void *mask, const void *bitval, const int64_t *index, int64_t n,
                      int32_t *bitmask)

@@ -94,6 +94,12 @@ void oracle_decompress_fetch_clustered(int w, const void *comp, int64_t num_vecs
                                        const void *masks, const void *bitvals, const int32_t *clusters,
                                        const int32_t *bitmask, const int64_t *index, int64_t n, void *out);
 
+/* oracle_compress_clustered on `threads` OpenMP threads, 32 rows per task
+ * (same output).  Used to time the host-side compression path. */
+void oracle_compress_mt(int w, const void *input, void *output, int64_t num_vecs, int64_t vec_size,
+                        const void *masks, const void *bitvals, const int32_t *clusters, const int64_t *index,
+                        int64_t n, int32_t *bitmask, int threads);
+
 /* src/compress.cu:238-313 + include/compress/ibp_compress_kernel.cuh:59-94
  * (compress_condensed): rows index[i] (or i) stored back to back, row i in
  * bytes [offsets[i-1], offsets[i]) (offsets[-1] = 0) -- the inclusive scan of

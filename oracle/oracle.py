@@ -57,6 +57,8 @@ def lib():
         L.oracle_compress_condensed.argtypes = [ctypes.c_int, _P, _I64, _I64, _P, _P, _P, _P, _I64, _P, _P]
         L.oracle_compress_condensed.restype = ctypes.c_int64
         L.oracle_decompress_condensed.argtypes = [ctypes.c_int, _P, _P, _I64, _I64, _P, _P, _P, _P, _I64, _P]
+        L.oracle_compress_mt.argtypes = [ctypes.c_int, _P, _P, _I64, _I64, _P, _P, _P, _P, _I64, _P,
+                                         ctypes.c_int]
         L.oracle_random_shuffle.argtypes = [_P, _I64]
         L.oracle_kmeans.argtypes = [ctypes.c_int, _P, _I64, _I64, ctypes.c_int, _I64, _P, _P, _P]
         L.oracle_sweep_last.argtypes = [ctypes.POINTER(ctypes.c_int)]
@@ -166,6 +168,19 @@ def compress(data, mask, bitval, index=None, clusters=None):
     bitmask = np.zeros((n + 31) // 32, dtype=np.int32)
     lib().oracle_compress_clustered(data.itemsize, _ptr(data), _ptr(out), n_rows, v, _ptr(_c(mask)),
                                     _ptr(_c(bitval)), _ptr(_clusters(clusters)), _ptr(idx), n, _ptr(bitmask))
+    return out, bitmask
+
+
+def compress_mt(data, mask, bitval, threads, index=None, clusters=None):
+    """compress() on `threads` OpenMP threads (same result)."""
+    data = _c(data)
+    n_rows, v = data.shape
+    idx = None if index is None else _c(np.asarray(index, dtype=np.int64))
+    n = n_rows if idx is None else idx.shape[0]
+    out = np.zeros((n, v), dtype=data.dtype)
+    bitmask = np.zeros((n + 31) // 32, dtype=np.int32)
+    lib().oracle_compress_mt(data.itemsize, _ptr(data), _ptr(out), n_rows, v, _ptr(_c(mask)), _ptr(_c(bitval)),
+                             _ptr(_clusters(clusters)), _ptr(idx), n, _ptr(bitmask), int(threads))
     return out, bitmask
 
 

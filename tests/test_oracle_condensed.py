@@ -195,3 +195,20 @@ def test_bitlist_brute_force_small_words(oracle, w, V):
         assert [int(v) for v in comp[0]] == words
         assert bitlist_unpack(comp[0], mask, bitval, w, packed) == [int(v) for v in row]
         assert np.array_equal(oracle.decompress_fetch(comp, mask, bitval, bm)[0], row)
+
+
+def test_compress_mt_matches_compress(oracle):
+    """The multi-threaded host compress (timed by the bench) gives the same
+    slots and bitmask as the single-threaded one, with an index and clusters."""
+    rng = np.random.default_rng(12)
+    n, v, k = 1000, 50, 3
+    x = _dataset(rng, n, v, 8)
+    masks, bitvals = _kmeans_like_masks(oracle, x, k)
+    cl = rng.integers(0, k, size=n).astype(np.int32)
+    idx = rng.integers(0, n, size=333)
+    for kw in (dict(), dict(index=idx), dict(clusters=cl), dict(index=idx, clusters=cl)):
+        m, b = (masks, bitvals) if "clusters" in kw else (masks[0], bitvals[0])
+        a, abm = oracle.compress(x, m, b, **kw)
+        for t in (1, 3, 8):
+            c, cbm = oracle.compress_mt(x, m, b, t, **kw)
+            assert np.array_equal(a, c) and np.array_equal(abm, cbm), (kw.keys(), t)
