@@ -54,10 +54,15 @@ struct FetchParams {
 
 // Fetched rows are written once and not read back by the kernel: streaming
 // (non-temporal) stores keep them from displacing compressed rows in L2/MALL.
+// Building with IBP_PLAIN_STORES selects ordinary stores.
 template <typename T>
 __device__ __forceinline__ void store_row_word(T *p, T v)
 {
+#ifdef IBP_PLAIN_STORES
+    *p = v;
+#else
     __builtin_nontemporal_store(v, p);
+#endif
 }
 
 template <typename IdxT>
