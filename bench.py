@@ -30,6 +30,11 @@ sys.path.insert(0, ROOT)
 METRIC = "decompressed GB/s into HBM (batched vs plain, bit-exact) + p99 fetch latency"
 HBM_PEAK_GBPS = 8000.0
 
+# HBM bytes per launch of the plain fetch, measured with two rocprofv3 --pmc
+# passes (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0, factors from a 256 MiB device
+# copy), keyed by (rows, vec_size, batch); profiles/hbm_traffic_fetch_gnn_minibatch.md.
+MEASURED_TRAFFIC = {(2449029, 50, 200000): 50792 * 1024 * 2 + 86885 * 1024}
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
@@ -273,7 +278,8 @@ def run_gnn_minibatch(args):
                                "per step, decompress_fetch into a preallocated output" % (num_rows, v, num_rows, batch),
                    "rows": num_rows, "vec_size": v, "batch": batch, "packed_rows": packed, "comp_len": comp_len},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBPS, 4),
+                     "traffic": MEASURED_TRAFFIC.get((num_rows, v, batch)),
                      "algorithmic_bytes_per_launch": traffic_alg, "kernel_ms": round(kern_ms, 5)},
         "cpu_baseline": cpu,
         "bitexact": bitexact and (batched is None or batched["bitexact"]) and (pcie is None or pcie["bitexact"]),
@@ -409,6 +415,15 @@ def run_serve(args):
     torch.cuda.synchronize()
     srv = ibp.Server(hbm_budget_bytes=8 << 30, num_streams=4, max_batch_requests=64, max_batch_rows=1 << 21)
     sids = [srv.register(s["comp"], s["mask"], s["bitval"], s["bitmask"], s["comp_len"]) for s in sets]
+    # first launches load kernels and set up streams: keep that out of the latencies
+    for sid, s in zip(sids, sets):
+        w = np.arange(1000) % s["x"].shape[0]
+        h = srv.submit(sid, w)
+        srv.wait(h)
+        srv.release(h)
+        ibp.decompress_fetch(s["comp"], s["mask"], s["bitval"], s["bitmask"], torch.device("cuda"), s["comp_len"],
+                             torch.from_numpy(w).cuda())
+    torch.cuda.synchronize()
     rng = np.random.default_rng(300)
     results = []
     for rate in list(args.rates) + [float("inf")]:
