@@ -81,7 +81,7 @@ __global__ __launch_bounds__(256) void batch_kernel(const TaskDev *tasks, int nt
     TaskDev t;
     SmallCols<uint64_t> c64;
     SmallCols<uint32_t> c32;
-    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int64_t wave = global_wave_id();
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
     int64_t next = wave; // static mode: next item of this wave
     for (;;) {

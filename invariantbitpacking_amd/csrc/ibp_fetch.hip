@@ -19,7 +19,7 @@ template <typename T, typename IdxT, int ROWS>
 __global__ __launch_bounds__(256) void fetch_small_kernel(FetchParams p)
 {
     const SmallCols<T> col = load_small_cols<T>(p.mask, p.bitval, (int)p.vec_size);
-    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int64_t wave = global_wave_id();
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
     fetch_small_range<T, IdxT, ROWS>(p, col, wave * ROWS, p.n, nwaves * ROWS);
 }
@@ -29,7 +29,7 @@ template <typename T, typename IdxT, int ROWS, int LAYOUT>
 __global__ __launch_bounds__(256) void fetch_small_meta_kernel(FetchParams p)
 {
     const SmallCols<T> col = load_small_cols<T>(p.mask, p.bitval, (int)p.vec_size);
-    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int64_t wave = global_wave_id();
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
     fetch_small_range_meta<T, IdxT, ROWS, LAYOUT>(p, col, wave * ROWS, p.n, nwaves * ROWS);
 }
@@ -37,7 +37,7 @@ __global__ __launch_bounds__(256) void fetch_small_meta_kernel(FetchParams p)
 template <typename T, typename IdxT, int LAYOUT>
 __global__ __launch_bounds__(256) void fetch_large_kernel(FetchParams p)
 {
-    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int64_t wave = global_wave_id();
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
     fetch_large_range<T, IdxT, LAYOUT>(p, wave, p.n, nwaves);
 }
@@ -52,7 +52,7 @@ template <typename T, typename IdxT>
 __global__ __launch_bounds__(256) void fetch_baseline_kernel(FetchParams p, int mode, int comp_len)
 {
     const int lane = lane_id();
-    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int64_t wave = global_wave_id();
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
     const int64_t V = p.vec_size;
     for (int64_t i = wave; i < p.n; i += nwaves) {

@@ -26,6 +26,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "ibp_bits.h"
 #include "ibp_wave.h"
 
@@ -128,16 +130,34 @@ __device__ __forceinline__ SmallCols<T> load_small_cols(const void *mask, const 
 }
 
 // Free bits of a flagged element (top-aligned in v) dropped into the zero
-// positions of the column mask.
-template <typename T>
+// positions of the column mask.  RUNS is the wave's run count (lanes with
+// fewer runs have zero entries), or -1 for the generic loop.
+template <typename T, int RUNS>
 __device__ __forceinline__ T small_deposit(const SmallCols<T> &c, T v)
 {
-    if (c.wave_runs > kRunCap) return deposit_msb<T>(v, c.free);
-    T d = T(0);
+    if constexpr (RUNS < 0) {
+        return deposit_msb<T>(v, c.free);
+    } else {
+        T d = T(0);
 #pragma unroll
-    for (int k = 0; k < kRunCap; ++k)
-        if (k < c.wave_runs) d |= T(v >> c.shift[k]) & c.run[k];
-    return d;
+        for (int k = 0; k < RUNS; ++k) d |= T(v >> c.shift[k]) & c.run[k];
+        return d;
+    }
+}
+
+// Calls f(std::integral_constant<int, RUNS>) for the wave's run count, so the
+// row loops are compiled once per run count with no per-row selection.
+template <typename T, typename F>
+__device__ __forceinline__ void with_runs(const SmallCols<T> &c, F &&f)
+{
+    switch (c.wave_runs) {
+    case 0: f(std::integral_constant<int, 0>()); break;
+    case 1: f(std::integral_constant<int, 1>()); break;
+    case 2: f(std::integral_constant<int, 2>()); break;
+    case 3: f(std::integral_constant<int, 3>()); break;
+    case 4: f(std::integral_constant<int, 4>()); break;
+    default: f(std::integral_constant<int, -1>()); break;
+    }
 }
 
 // Header flag of lane `lane` for a row of <= 64 elements, given the first
@@ -227,12 +247,14 @@ __device__ __forceinline__ RowMeta decode_meta(const FetchParams &p, int64_t s, 
 // below end.  Per group, every row's slot read is issued at once (a prefix of
 // first_words words, or the whole raw row) in parallel with the bitmask words,
 // and the next group's indices are prefetched, so a wave keeps ROWS rows and
-// one index load in flight.  Wave-uniform.
-template <typename T, typename IdxT, int ROWS>
-__device__ __forceinline__ void fetch_small_range(const FetchParams &p, const SmallCols<T> &col, int64_t begin,
-                                                  int64_t end, int64_t step)
+// one index load in flight.  Row pointers are wave-uniform (scalar base +
+// lane offset addressing) and bit positions unsigned.  Wave-uniform.
+template <typename T, typename IdxT, int ROWS, int RUNS>
+__device__ __forceinline__ void fetch_small_range_r(const FetchParams &p, const SmallCols<T> &col, int64_t begin,
+                                                    int64_t end, int64_t step)
 {
-    constexpr int B = Bits<T>::B;
+    constexpr unsigned B = (unsigned)Bits<T>::B;
+    constexpr unsigned LOGB = sizeof(T) == 8 ? 6 : sizeof(T) == 4 ? 5 : sizeof(T) == 2 ? 4 : 3;
     const int lane = lane_id();
     const int V = (int)p.vec_size;
     const int Hw = (int)(header_bytes(V, (int)sizeof(T)) / (int)sizeof(T));
@@ -250,13 +272,13 @@ __device__ __forceinline__ void fetch_small_range(const FetchParams &p, const Sm
         const uint64_t okmask = ballot(my_ok);
 
         // Stage 1: slot prefixes of every row, bitmask words, next indices.
-        int64_t src[ROWS];
+        const T *row[ROWS];
         T w[ROWS];
 #pragma unroll
         for (int r = 0; r < ROWS; ++r) {
-            src[r] = readlane(cur, r);
+            row[r] = comp + readlane(cur, r) * V;
             w[r] = T(0);
-            if (((okmask >> r) & 1ull) && lane < E) w[r] = comp[src[r] * V + lane];
+            if (((okmask >> r) & 1ull) && lane < E) w[r] = row[r][lane];
         }
         uint32_t my_bm = 0;
         if (my_ok) my_bm = p.bitmask[cur >> 5];
@@ -268,7 +290,7 @@ __device__ __forceinline__ void fetch_small_range(const FetchParams &p, const Sm
         const uint64_t pkmask = ballot(my_ok && ((my_bm >> (cur & 31)) & 1u));
 
         // Stage 2: header -> flags -> bit offsets -> words needed; read the rest.
-        int pos[ROWS];
+        unsigned pos[ROWS];
         bool flag[ROWS];
         int need[ROWS];
 #pragma unroll
@@ -279,36 +301,38 @@ __device__ __forceinline__ void fetch_small_range(const FetchParams &p, const Sm
             pos[r] = 0;
             need[r] = ok ? V : 0;
             if (packed) { // wave-uniform
-                flag[r] = col.valid && small_flag<T>(w[r], lane);
-                const int L = col.valid ? (flag[r] ? col.len_flagged : B) : 0;
+                const bool f = small_flag<T>(w[r], lane);
+                flag[r] = col.valid && f;
+                const int L = col.valid ? (flag[r] ? col.len_flagged : (int)B) : 0;
                 const int incl = wave_incl_scan(L);
-                const int total = readlane(incl, kWave - 1);
-                pos[r] = incl - L;
+                const unsigned total = (unsigned)readlane(incl, kWave - 1);
+                pos[r] = (unsigned)(incl - L);
                 // Clamped to the slot: a corrupt header cannot make the read leave the row.
-                const int nw = Hw + (total + B - 1) / B;
+                const int nw = Hw + (int)((total + B - 1) >> LOGB);
                 need[r] = nw < V ? nw : V;
             }
         }
 #pragma unroll
         for (int r = 0; r < ROWS; ++r)
-            if (need[r] > E && lane >= E && lane < need[r]) w[r] = comp[src[r] * V + lane];
+            if (need[r] > E && lane >= E && lane < need[r]) w[r] = row[r][lane];
 
         // Stage 3: unpack and store.
+        T *orow = out + base * V;
 #pragma unroll
         for (int r = 0; r < ROWS; ++r) {
-            const int64_t i = base + r;
             const bool ok = (okmask >> r) & 1ull;
             const bool packed = (pkmask >> r) & 1ull;
             T val = w[r];
             if (packed) {
-                const int q = Hw + pos[r] / B;
-                const int o = pos[r] % B;
+                const int q = Hw + (int)(pos[r] >> LOGB);
+                const int o = (int)(pos[r] & (B - 1));
                 const T w0 = shfl(w[r], q);
                 const T w1 = shfl(w[r], q + 1 < kWave ? q + 1 : kWave - 1);
                 const T v = stream_bits<T>(w0, w1, o);
-                val = flag[r] ? T(col.bitval | small_deposit<T>(col, v)) : v;
+                val = flag[r] ? T(col.bitval | small_deposit<T, RUNS>(col, v)) : v;
             }
-            if (i < end && lane < V) store_row_word<T>(&out[i * V + lane], ok ? val : T(0));
+            if (base + r < end && lane < V) store_row_word<T>(&orow[lane], ok ? val : T(0));
+            orow += V;
         }
         // The ballot needs the whole wave active, so it is taken before the lane-0 branch.
         const uint64_t livemask = ballot(live);
@@ -317,6 +341,15 @@ __device__ __forceinline__ void fetch_small_range(const FetchParams &p, const Sm
             if (bad) atomicAdd(p.err, (unsigned long long)bad);
         }
     }
+}
+
+template <typename T, typename IdxT, int ROWS>
+__device__ __forceinline__ void fetch_small_range(const FetchParams &p, const SmallCols<T> &col, int64_t begin,
+                                                  int64_t end, int64_t step)
+{
+    with_runs<T>(col, [&](auto runs) {
+        fetch_small_range_r<T, IdxT, ROWS, decltype(runs)::value>(p, col, begin, end, step);
+    });
 }
 
 // fetch_small_range for the condensed and clustered layouts.  A row's
@@ -425,7 +458,7 @@ __device__ __forceinline__ void fetch_small_range_meta(const FetchParams &p, con
                 if constexpr (CLUS)
                     val = flag[r] ? T(cb[r] | deposit_msb<T>(v, T(~cm[r]))) : v;
                 else
-                    val = flag[r] ? T(col.bitval | small_deposit<T>(col, v)) : v;
+                    val = flag[r] ? T(col.bitval | small_deposit<T, -1>(col, v)) : v;
             }
             if (i < end && lane < V) store_row_word<T>(&out[i * V + lane], ok ? val : T(0));
         }

@@ -48,7 +48,7 @@ __global__ __launch_bounds__(256) void km_assign_kernel(const T *data, int64_t N
                                                         int K, int32_t *cluster)
 {
     const int lane = lane_id();
-    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int64_t wave = global_wave_id();
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
     for (int64_t i = wave; i < N; i += nwaves) {
         int best = 0;

@@ -58,7 +58,7 @@ __global__ __launch_bounds__(256) void size_kernel(const T *data, int64_t num_ve
                                                    unsigned long long *err)
 {
     const int lane = lane_id();
-    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int64_t wave = global_wave_id();
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
     unsigned long long acc = 0, bad = 0;
     for (int64_t i = wave; i < n; i += nwaves) {
@@ -138,8 +138,8 @@ __global__ __launch_bounds__(256) void pack_kernel(PackParams p)
     constexpr int W = (int)sizeof(T);
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const int lane = lane_id();
-    const int wave_in_block = threadIdx.x / kWave;
-    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int wave_in_block = wave_in_block_id();
+    const int64_t wave = global_wave_id();
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
     const int64_t V = p.V;
     const int64_t buf_bytes = slot_buf_bytes(V, W);

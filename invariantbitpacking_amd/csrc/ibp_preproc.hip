@@ -37,7 +37,7 @@ __global__ __launch_bounds__(256) void count_bits_kernel(const T *data, int64_t 
 {
     constexpr int B = 8 * (int)sizeof(T);
     const int lane = lane_id();
-    const int wave_in_block = threadIdx.x / kWave;
+    const int wave_in_block = wave_in_block_id();
     const int waves = blockDim.x / kWave;
     const int64_t j = (int64_t)blockIdx.y * kWave + lane;
     const int64_t r0 = (int64_t)blockIdx.x * rows_per_block;
@@ -102,7 +102,7 @@ __global__ __launch_bounds__(256) void score_kernel(const T *data, int64_t num_v
                                                     const T *bitvals, int nt, unsigned long long *scores)
 {
     const int lane = lane_id();
-    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int64_t wave = global_wave_id();
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
     unsigned long long saved[NT];
 #pragma unroll

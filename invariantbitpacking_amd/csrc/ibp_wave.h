@@ -12,6 +12,15 @@ constexpr int kWave = 64;
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (kWave - 1)); }
 
+// Index of the calling wave within its block and across the grid.  Computed
+// through readfirstlane so the compiler keeps them (and every row index
+// derived from them) in scalar registers.
+__device__ __forceinline__ int wave_in_block_id() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave)); }
+__device__ __forceinline__ int64_t global_wave_id()
+{
+    return (int64_t)blockIdx.x * (int64_t)(blockDim.x / kWave) + wave_in_block_id();
+}
+
 // Inclusive prefix sum over the 64 lanes (int32), DPP row shifts within each
 // row of 16 lanes followed by the two row broadcasts of GFX9-family DPP.
 __device__ __forceinline__ int wave_incl_scan(int x)
