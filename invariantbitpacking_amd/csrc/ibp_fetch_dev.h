@@ -242,6 +242,75 @@ __device__ __forceinline__ RowMeta decode_meta(const FetchParams &p, int64_t s, 
     return m;
 }
 
+// Stages 2 and 3 of fetch_small_range_r for one group of rows whose slot
+// prefixes are in w: header -> flags -> bit offsets -> words needed, the rest
+// of the rows read, then unpack and store.  ALL_PACKED: every row of the
+// group is valid and packed, so nothing branches per row.
+template <typename T, int ROWS, int RUNS, bool ALL_PACKED>
+__device__ __forceinline__ void small_unpack_rows(const FetchParams &p, const SmallCols<T> &col, T (&w)[ROWS],
+                                                  int64_t cur, int64_t base, int nrows, uint64_t okmask,
+                                                  uint64_t pkmask, int len_flag, int len_raw, int Hw, int E,
+                                                  const T *comp, T *out)
+{
+    constexpr unsigned B = (unsigned)Bits<T>::B;
+    constexpr unsigned LOGB = sizeof(T) == 8 ? 6 : sizeof(T) == 4 ? 5 : sizeof(T) == 2 ? 4 : 3;
+    const int lane = lane_id();
+    const int V = (int)p.vec_size;
+    const unsigned hw_addr = (unsigned)Hw << 2;
+
+    unsigned pos[ROWS];
+    bool flag[ROWS];
+    int need[ROWS];
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+        flag[r] = false;
+        pos[r] = 0;
+        need[r] = (ALL_PACKED || ((okmask >> r) & 1ull)) ? V : 0;
+        if (ALL_PACKED || ((pkmask >> r) & 1ull)) { // wave-uniform
+            flag[r] = small_flag<T>(w[r], lane) && col.valid;
+            const int L = flag[r] ? len_flag : len_raw;
+            const int incl = wave_incl_scan(L);
+            const unsigned total = (unsigned)readlane(incl, kWave - 1);
+            pos[r] = (unsigned)(incl - L);
+            // Clamped to the slot: a corrupt header cannot make the read leave the row.
+            const int nw = Hw + (int)((total + B - 1) >> LOGB);
+            need[r] = nw < V ? nw : V;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r)
+        if (need[r] > E) { // uniform; rare for packed rows
+            const T *row = comp + readlane(cur, r) * V;
+            if (lane >= E && lane < need[r]) w[r] = row[lane];
+        }
+    // Every load of the group (and the index prefetch) lands here, once:
+    // the stores below then wait on nothing (vmcnt counts stores too).
+    __builtin_amdgcn_s_waitcnt(0x0F70); // vmcnt(0) expcnt(7) lgkmcnt(15)
+
+    // Unpack and store.  Payload bits never pass the slot's last word, so
+    // q + 1 needs no clamp (ds_bpermute stays inside the wave).
+    T *orow = out + base * V;
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+        if (ALL_PACKED || r < nrows) {
+            T val = T(0);
+            if (ALL_PACKED || ((okmask >> r) & 1ull)) {
+                val = w[r];
+                if (ALL_PACKED || ((pkmask >> r) & 1ull)) {
+                    const unsigned a0 = hw_addr + ((pos[r] >> LOGB) << 2);
+                    const int o = (int)(pos[r] & (B - 1));
+                    const T w0 = shfl(w[r], (int)(a0 >> 2));
+                    const T w1 = shfl(w[r], (int)(a0 >> 2) + 1);
+                    const T v = stream_bits<T>(w0, w1, o);
+                    val = flag[r] ? T(col.bitval | small_deposit<T, RUNS>(col, v)) : v;
+                }
+            }
+            if (lane < V) store_row_word<T>(&orow[lane], val);
+        }
+        orow += V;
+    }
+}
+
 // Fetches and unpacks rows of a table whose rows have at most 64 elements:
 // groups of ROWS consecutive output rows starting at begin, begin + step, ...
 // below end.  Per group, every row's slot read is issued at once (a prefix of
@@ -253,8 +322,6 @@ template <typename T, typename IdxT, int ROWS, int RUNS>
 __device__ __forceinline__ void fetch_small_range_r(const FetchParams &p, const SmallCols<T> &col, int64_t begin,
                                                     int64_t end, int64_t step)
 {
-    constexpr unsigned B = (unsigned)Bits<T>::B;
-    constexpr unsigned LOGB = sizeof(T) == 8 ? 6 : sizeof(T) == 4 ? 5 : sizeof(T) == 2 ? 4 : 3;
     const int lane = lane_id();
     const int V = (int)p.vec_size;
     const int Hw = (int)(header_bytes(V, (int)sizeof(T)) / (int)sizeof(T));
@@ -262,23 +329,29 @@ __device__ __forceinline__ void fetch_small_range_r(const FetchParams &p, const 
     const T *comp = (const T *)p.comp;
     T *out = (T *)p.out;
     const bool has_index = p.index != nullptr;
+    // Per-lane payload length of a flagged and of a raw element (zero past the row).
+    const int len_flag = col.valid ? col.len_flagged : 0;
+    const int len_raw = col.valid ? Bits<T>::B : 0;
 
     int64_t nxt = 0;
     if (has_index && lane < ROWS && begin + lane < end) nxt = load_index<IdxT>(p.index, begin + lane);
     for (int64_t base = begin; base < end; base += step) {
-        const bool live = lane < ROWS && base + lane < end;
+        // Rows of this group (uniform): the last group may be partial.
+        const int nrows = end - base < ROWS ? (int)(end - base) : ROWS;
+        const bool live = lane < nrows;
         const int64_t cur = has_index ? nxt : base + lane;
         const bool my_ok = live && cur >= 0 && cur < p.num_vecs;
         const uint64_t okmask = ballot(my_ok);
 
         // Stage 1: slot prefixes of every row, bitmask words, next indices.
-        const T *row[ROWS];
         T w[ROWS];
 #pragma unroll
         for (int r = 0; r < ROWS; ++r) {
-            row[r] = comp + readlane(cur, r) * V;
             w[r] = T(0);
-            if (((okmask >> r) & 1ull) && lane < E) w[r] = row[r][lane];
+            if ((okmask >> r) & 1ull) {
+                const T *row = comp + readlane(cur, r) * V;
+                if (lane < E) w[r] = row[lane];
+            }
         }
         uint32_t my_bm = 0;
         if (my_ok) my_bm = p.bitmask[cur >> 5];
@@ -289,51 +362,20 @@ __device__ __forceinline__ void fetch_small_range_r(const FetchParams &p, const 
         }
         const uint64_t pkmask = ballot(my_ok && ((my_bm >> (cur & 31)) & 1u));
 
-        // Stage 2: header -> flags -> bit offsets -> words needed; read the rest.
-        unsigned pos[ROWS];
-        bool flag[ROWS];
-        int need[ROWS];
-#pragma unroll
-        for (int r = 0; r < ROWS; ++r) {
-            const bool ok = (okmask >> r) & 1ull;
-            const bool packed = (pkmask >> r) & 1ull;
-            flag[r] = false;
-            pos[r] = 0;
-            need[r] = ok ? V : 0;
-            if (packed) { // wave-uniform
-                const bool f = small_flag<T>(w[r], lane);
-                flag[r] = col.valid && f;
-                const int L = col.valid ? (flag[r] ? col.len_flagged : (int)B) : 0;
-                const int incl = wave_incl_scan(L);
-                const unsigned total = (unsigned)readlane(incl, kWave - 1);
-                pos[r] = (unsigned)(incl - L);
-                // Clamped to the slot: a corrupt header cannot make the read leave the row.
-                const int nw = Hw + (int)((total + B - 1) >> LOGB);
-                need[r] = nw < V ? nw : V;
-            }
-        }
-#pragma unroll
-        for (int r = 0; r < ROWS; ++r)
-            if (need[r] > E && lane >= E && lane < need[r]) w[r] = row[r][lane];
-
-        // Stage 3: unpack and store.
-        T *orow = out + base * V;
-#pragma unroll
-        for (int r = 0; r < ROWS; ++r) {
-            const bool ok = (okmask >> r) & 1ull;
-            const bool packed = (pkmask >> r) & 1ull;
-            T val = w[r];
-            if (packed) {
-                const int q = Hw + (int)(pos[r] >> LOGB);
-                const int o = (int)(pos[r] & (B - 1));
-                const T w0 = shfl(w[r], q);
-                const T w1 = shfl(w[r], q + 1 < kWave ? q + 1 : kWave - 1);
-                const T v = stream_bits<T>(w0, w1, o);
-                val = flag[r] ? T(col.bitval | small_deposit<T, RUNS>(col, v)) : v;
-            }
-            if (base + r < end && lane < V) store_row_word<T>(&orow[lane], ok ? val : T(0));
-            orow += V;
-        }
+        // Stages 2 and 3.  A full group whose rows are all valid and packed
+        // (the common case) runs without per-row branches, so the rows' scans
+        // and exchanges interleave; any other group takes the general form.
+        const uint64_t all = nrows == ROWS ? ((1ull << ROWS) - 1) : 0ull;
+#ifdef IBP_NO_BRANCH_FREE_GROUPS
+        if (false)
+#else
+        if (all && (okmask & pkmask & all) == all)
+#endif
+            small_unpack_rows<T, ROWS, RUNS, true>(p, col, w, cur, base, ROWS, okmask, pkmask, len_flag, len_raw, Hw, E,
+                                                   comp, out);
+        else
+            small_unpack_rows<T, ROWS, RUNS, false>(p, col, w, cur, base, nrows, okmask, pkmask, len_flag, len_raw, Hw,
+                                                    E, comp, out);
         // The ballot needs the whole wave active, so it is taken before the lane-0 branch.
         const uint64_t livemask = ballot(live);
         if (p.err && lane == 0) {
