@@ -13,8 +13,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(HERE)
 OUT = os.path.join(PKG, "libibp.so")
 OBJ_DIR = os.path.join(PKG, "_build")
-SOURCES = ["ibp_core.hip", "ibp_fetch.hip", "ibp_pack.hip", "ibp_preproc.hip", "ibp_kmeans.hip",
-           "ibp_batch.hip", "ibp_serve.hip", "ibp_io.hip"]
+SOURCES = ["ibp_core.hip", "ibp_fetch.hip", "ibp_fetch_u64.hip", "ibp_fetch_u32.hip", "ibp_fetch_narrow.hip",
+           "ibp_pack.hip", "ibp_preproc.hip", "ibp_kmeans.hip", "ibp_batch.hip", "ibp_serve.hip", "ibp_io.hip"]
 ARCH = os.environ.get("IBP_OFFLOAD_ARCH", "gfx950")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
          # packed FP32 instructions are kept out of every HIP build

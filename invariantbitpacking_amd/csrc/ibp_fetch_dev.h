@@ -54,17 +54,25 @@ struct FetchParams {
     int64_t num_clusters = 1;          // clustered: rows of mask/bitval
 };
 
-// Fetched rows are written once and not read back by the kernel: streaming
-// (non-temporal) stores keep them from displacing compressed rows in L2/MALL.
-// Building with IBP_PLAIN_STORES selects ordinary stores.
-template <typename T>
+// Output stores.  Rows that are a whole number of 128-byte lines are written
+// with non-temporal stores: nothing is written twice and the outputs do not
+// evict hot compressed rows from L2/MALL (DLRM, 512-byte rows: 0.385 vs
+// 0.466 ms per batch).  Other rows share a line with their neighbour; there
+// ordinary stores let L2 merge the halves, where non-temporal ones would write
+// the line twice (GNN minibatch, 400-byte rows: 6% faster).
+template <typename T, bool NT>
 __device__ __forceinline__ void store_row_word(T *p, T v)
 {
-#ifdef IBP_PLAIN_STORES
-    *p = v;
-#else
-    __builtin_nontemporal_store(v, p);
-#endif
+    if constexpr (NT)
+        __builtin_nontemporal_store(v, p);
+    else
+        *p = v;
+}
+
+// Whether rows of this table take non-temporal stores (see store_row_word).
+__device__ __forceinline__ bool rows_fill_lines(int64_t vec_size, int elem_bytes)
+{
+    return ((vec_size * elem_bytes) & 127) == 0;
 }
 
 template <typename IdxT>
@@ -145,18 +153,25 @@ __device__ __forceinline__ T small_deposit(const SmallCols<T> &c, T v)
     }
 }
 
-// Calls f(std::integral_constant<int, RUNS>) for the wave's run count, so the
-// row loops are compiled once per run count with no per-row selection.
+// Calls f(std::integral_constant<int, RUNS>, std::bool_constant<NT>) for the
+// wave's run count and the table's store policy, so the row loops are
+// compiled once per combination with no per-row selection.
 template <typename T, typename F>
-__device__ __forceinline__ void with_runs(const SmallCols<T> &c, F &&f)
+__device__ __forceinline__ void with_runs(const SmallCols<T> &c, bool nt, F &&f)
 {
+    auto go = [&](auto runs) {
+        if (nt)
+            f(runs, std::true_type());
+        else
+            f(runs, std::false_type());
+    };
     switch (c.wave_runs) {
-    case 0: f(std::integral_constant<int, 0>()); break;
-    case 1: f(std::integral_constant<int, 1>()); break;
-    case 2: f(std::integral_constant<int, 2>()); break;
-    case 3: f(std::integral_constant<int, 3>()); break;
-    case 4: f(std::integral_constant<int, 4>()); break;
-    default: f(std::integral_constant<int, -1>()); break;
+    case 0: go(std::integral_constant<int, 0>()); break;
+    case 1: go(std::integral_constant<int, 1>()); break;
+    case 2: go(std::integral_constant<int, 2>()); break;
+    case 3: go(std::integral_constant<int, 3>()); break;
+    case 4: go(std::integral_constant<int, 4>()); break;
+    default: go(std::integral_constant<int, -1>()); break;
     }
 }
 
@@ -246,7 +261,7 @@ __device__ __forceinline__ RowMeta decode_meta(const FetchParams &p, int64_t s, 
 // prefixes are in w: header -> flags -> bit offsets -> words needed, the rest
 // of the rows read, then unpack and store.  ALL_PACKED: every row of the
 // group is valid and packed, so nothing branches per row.
-template <typename T, int ROWS, int RUNS, bool ALL_PACKED>
+template <typename T, int ROWS, int RUNS, bool NT, bool ALL_PACKED>
 __device__ __forceinline__ void small_unpack_rows(const FetchParams &p, const SmallCols<T> &col, T (&w)[ROWS],
                                                   int64_t cur, int64_t base, int nrows, uint64_t okmask,
                                                   uint64_t pkmask, int len_flag, int len_raw, int Hw, int E,
@@ -305,7 +320,7 @@ __device__ __forceinline__ void small_unpack_rows(const FetchParams &p, const Sm
                     val = flag[r] ? T(col.bitval | small_deposit<T, RUNS>(col, v)) : v;
                 }
             }
-            if (lane < V) store_row_word<T>(&orow[lane], val);
+            if (lane < V) store_row_word<T, NT>(&orow[lane], val);
         }
         orow += V;
     }
@@ -318,7 +333,7 @@ __device__ __forceinline__ void small_unpack_rows(const FetchParams &p, const Sm
 // and the next group's indices are prefetched, so a wave keeps ROWS rows and
 // one index load in flight.  Row pointers are wave-uniform (scalar base +
 // lane offset addressing) and bit positions unsigned.  Wave-uniform.
-template <typename T, typename IdxT, int ROWS, int RUNS>
+template <typename T, typename IdxT, int ROWS, int RUNS, bool NT>
 __device__ __forceinline__ void fetch_small_range_r(const FetchParams &p, const SmallCols<T> &col, int64_t begin,
                                                     int64_t end, int64_t step)
 {
@@ -371,10 +386,10 @@ __device__ __forceinline__ void fetch_small_range_r(const FetchParams &p, const 
 #else
         if (all && (okmask & pkmask & all) == all)
 #endif
-            small_unpack_rows<T, ROWS, RUNS, true>(p, col, w, cur, base, ROWS, okmask, pkmask, len_flag, len_raw, Hw, E,
+            small_unpack_rows<T, ROWS, RUNS, NT, true>(p, col, w, cur, base, ROWS, okmask, pkmask, len_flag, len_raw, Hw, E,
                                                    comp, out);
         else
-            small_unpack_rows<T, ROWS, RUNS, false>(p, col, w, cur, base, nrows, okmask, pkmask, len_flag, len_raw, Hw,
+            small_unpack_rows<T, ROWS, RUNS, NT, false>(p, col, w, cur, base, nrows, okmask, pkmask, len_flag, len_raw, Hw,
                                                     E, comp, out);
         // The ballot needs the whole wave active, so it is taken before the lane-0 branch.
         const uint64_t livemask = ballot(live);
@@ -389,8 +404,8 @@ template <typename T, typename IdxT, int ROWS>
 __device__ __forceinline__ void fetch_small_range(const FetchParams &p, const SmallCols<T> &col, int64_t begin,
                                                   int64_t end, int64_t step)
 {
-    with_runs<T>(col, [&](auto runs) {
-        fetch_small_range_r<T, IdxT, ROWS, decltype(runs)::value>(p, col, begin, end, step);
+    with_runs<T>(col, rows_fill_lines(p.vec_size, (int)sizeof(T)), [&](auto runs, auto nt) {
+        fetch_small_range_r<T, IdxT, ROWS, decltype(runs)::value, decltype(nt)::value>(p, col, begin, end, step);
     });
 }
 
@@ -400,7 +415,7 @@ __device__ __forceinline__ void fetch_small_range(const FetchParams &p, const Sm
 // per row the column constants are loaded per row and the generic deposit is
 // used.  Rows whose index, byte range or mask row is invalid give zero rows
 // and are counted.  Wave-uniform.
-template <typename T, typename IdxT, int ROWS, int LAYOUT>
+template <typename T, typename IdxT, int ROWS, int LAYOUT, bool NT, int RUNS>
 __device__ __forceinline__ void fetch_small_range_meta(const FetchParams &p, const SmallCols<T> &col,
                                                        int64_t begin, int64_t end, int64_t step)
 {
@@ -500,9 +515,9 @@ __device__ __forceinline__ void fetch_small_range_meta(const FetchParams &p, con
                 if constexpr (CLUS)
                     val = flag[r] ? T(cb[r] | deposit_msb<T>(v, T(~cm[r]))) : v;
                 else
-                    val = flag[r] ? T(col.bitval | small_deposit<T, -1>(col, v)) : v;
+                    val = flag[r] ? T(col.bitval | small_deposit<T, RUNS>(col, v)) : v;
             }
-            if (i < end && lane < V) store_row_word<T>(&out[i * V + lane], ok ? val : T(0));
+            if (i < end && lane < V) store_row_word<T, NT>(&out[i * V + lane], ok ? val : T(0));
         }
         const uint64_t livemask = ballot(live);
         if (p.err && lane == 0) {
@@ -548,7 +563,7 @@ __device__ __forceinline__ void fetch_large_row_from(const FetchParams &p, int64
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int64_t k = k0 + u * kWave;
-                if (k < V) store_row_word<T>(&dst[k], a[u]);
+                if (k < V) store_row_word<T, true>(&dst[k], a[u]);
             }
         }
         return;
@@ -601,7 +616,7 @@ __device__ __forceinline__ void fetch_large_row_from(const FetchParams &p, int64
         const T a1 = q + 1 < kWave ? s1 : X;
         const T v = stream_bits<T>(a0, a1, o);
         const T val = f ? T(b | deposit_msb<T>(v, T(~m))) : v;
-        if (valid) store_row_word<T>(&dst[j], val);
+        if (valid) store_row_word<T, true>(&dst[j], val);
         cursor += tot;
     }
 }
