@@ -419,7 +419,8 @@ template <typename T, typename IdxT, int ROWS, int LAYOUT, bool NT, int RUNS>
 __device__ __forceinline__ void fetch_small_range_meta(const FetchParams &p, const SmallCols<T> &col,
                                                        int64_t begin, int64_t end, int64_t step)
 {
-    constexpr int B = Bits<T>::B;
+    constexpr unsigned B = (unsigned)Bits<T>::B;
+    constexpr unsigned LOGB = sizeof(T) == 8 ? 6 : sizeof(T) == 4 ? 5 : sizeof(T) == 2 ? 4 : 3;
     constexpr bool CLUS = (LAYOUT & kLayoutClustered) != 0;
     const int lane = lane_id();
     const int V = (int)p.vec_size;
@@ -430,6 +431,8 @@ __device__ __forceinline__ void fetch_small_range_meta(const FetchParams &p, con
     const T *bitval = (const T *)p.bitval;
     T *out = (T *)p.out;
     const bool colv = lane < V;
+    const int len_raw = colv ? (int)B : 0;
+    const unsigned hw_addr = (unsigned)Hw << 2;
 
     auto live_at = [&](int64_t b) { return lane < ROWS && b + lane < end; };
     auto id_at = [&](int64_t b) -> int64_t { return live_at(b) ? load_index<IdxT>(p.index, b + lane) : 0; };
@@ -438,30 +441,28 @@ __device__ __forceinline__ void fetch_small_range_meta(const FetchParams &p, con
     RowMetaRaw craw = load_meta<LAYOUT>(p, cur, live_at(begin));
     int64_t nxt = id_at(begin + step);
     for (int64_t base = begin; base < end; base += step) {
-        const bool live = live_at(base);
+        const int nrows = end - base < ROWS ? (int)(end - base) : ROWS; // uniform
+        const bool live = lane < nrows;
         const RowMeta me = decode_meta<T, LAYOUT>(p, cur, live, craw);
         const uint64_t okmask = ballot(me.ok);
         const uint64_t pkmask = ballot(me.ok && me.packed);
 
         // Stage 1: stored-row prefixes (and mask rows), the next group's
         // metadata, the index of the group after it.
-        int64_t word[ROWS];
-        int len[ROWS];
         T w[ROWS], cm[ROWS], cb[ROWS];
 #pragma unroll
         for (int r = 0; r < ROWS; ++r) {
-            const bool ok = (okmask >> r) & 1ull;
-            word[r] = readlane(me.word, r);
-            len[r] = (int)readlane(me.len, r);
-            const int first = E < len[r] ? E : len[r];
-            w[r] = T(0);
-            if (ok && lane < first) w[r] = comp[word[r] + lane];
-            cm[r] = cb[r] = T(0);
-            if constexpr (CLUS) {
-                const int64_t mrow = (int64_t)readlane(me.cl, r) * V;
-                if (ok && colv) {
-                    cm[r] = mask[mrow + lane];
-                    cb[r] = bitval[mrow + lane];
+            w[r] = cm[r] = cb[r] = T(0);
+            if ((okmask >> r) & 1ull) { // uniform
+                const T *row = comp + readlane(me.word, r);
+                const int len = (int)readlane(me.len, r);
+                if (lane < (E < len ? E : len)) w[r] = row[lane];
+                if constexpr (CLUS) {
+                    const int64_t mrow = (int64_t)readlane(me.cl, r) * V;
+                    if (colv) {
+                        cm[r] = mask[mrow + lane];
+                        cb[r] = bitval[mrow + lane];
+                    }
                 }
             }
         }
@@ -469,55 +470,63 @@ __device__ __forceinline__ void fetch_small_range_meta(const FetchParams &p, con
         const int64_t nxt2 = id_at(base + 2 * step);
 
         // Stage 2: header -> flags -> bit offsets -> words needed; read the rest.
-        int pos[ROWS];
+        unsigned pos[ROWS];
         bool flag[ROWS];
         int need[ROWS];
 #pragma unroll
         for (int r = 0; r < ROWS; ++r) {
-            const bool ok = (okmask >> r) & 1ull;
-            const bool packed = (pkmask >> r) & 1ull;
             flag[r] = false;
             pos[r] = 0;
-            need[r] = ok ? len[r] : 0;
-            if (packed) { // wave-uniform
-                const bool f = small_flag<T>(w[r], lane);
-                flag[r] = colv && f;
+            need[r] = ((okmask >> r) & 1ull) ? (int)readlane(me.len, r) : 0;
+            if ((pkmask >> r) & 1ull) { // uniform
+                flag[r] = small_flag<T>(w[r], lane) && colv;
                 int lf;
                 if constexpr (CLUS)
-                    lf = B - popc(cm[r]);
+                    lf = (int)B - popc(cm[r]);
                 else
                     lf = col.len_flagged;
-                const int L = colv ? (flag[r] ? lf : B) : 0;
+                const int L = flag[r] ? lf : len_raw;
                 const int incl = wave_incl_scan(L);
-                const int total = readlane(incl, kWave - 1);
-                pos[r] = incl - L;
-                const int nw = Hw + (total + B - 1) / B;
-                need[r] = nw < len[r] ? nw : len[r];
+                const unsigned total = (unsigned)readlane(incl, kWave - 1);
+                pos[r] = (unsigned)(incl - L);
+                const int nw = Hw + (int)((total + B - 1) >> LOGB);
+                need[r] = nw < need[r] ? nw : need[r]; // never past the stored row
             }
         }
 #pragma unroll
         for (int r = 0; r < ROWS; ++r)
-            if (need[r] > E && lane >= E && lane < need[r]) w[r] = comp[word[r] + lane];
+            if (need[r] > E) { // uniform
+                const T *row = comp + readlane(me.word, r);
+                if (lane >= E && lane < need[r]) w[r] = row[lane];
+            }
+        // Every load of the group lands here, once: the stores below wait on
+        // nothing (vmcnt counts stores too).  The next group's metadata and
+        // index loads are issued before this point and land here as well.
+        __builtin_amdgcn_s_waitcnt(0x0F70); // vmcnt(0) expcnt(7) lgkmcnt(15)
 
         // Stage 3: unpack and store.
+        T *orow = out + base * V;
 #pragma unroll
         for (int r = 0; r < ROWS; ++r) {
-            const int64_t i = base + r;
-            const bool ok = (okmask >> r) & 1ull;
-            const bool packed = (pkmask >> r) & 1ull;
-            T val = w[r];
-            if (packed) {
-                const int q = Hw + pos[r] / B;
-                const int o = pos[r] % B;
-                const T w0 = shfl(w[r], q);
-                const T w1 = shfl(w[r], q + 1 < kWave ? q + 1 : kWave - 1);
-                const T v = stream_bits<T>(w0, w1, o);
-                if constexpr (CLUS)
-                    val = flag[r] ? T(cb[r] | deposit_msb<T>(v, T(~cm[r]))) : v;
-                else
-                    val = flag[r] ? T(col.bitval | small_deposit<T, RUNS>(col, v)) : v;
+            if (r < nrows) {
+                T val = T(0);
+                if ((okmask >> r) & 1ull) {
+                    val = w[r];
+                    if ((pkmask >> r) & 1ull) {
+                        const unsigned a0 = hw_addr + ((pos[r] >> LOGB) << 2);
+                        const int o = (int)(pos[r] & (B - 1));
+                        const T w0 = shfl(w[r], (int)(a0 >> 2));
+                        const T w1 = shfl(w[r], (int)(a0 >> 2) + 1);
+                        const T v = stream_bits<T>(w0, w1, o);
+                        if constexpr (CLUS)
+                            val = flag[r] ? T(cb[r] | deposit_msb<T>(v, T(~cm[r]))) : v;
+                        else
+                            val = flag[r] ? T(col.bitval | small_deposit<T, RUNS>(col, v)) : v;
+                    }
+                }
+                if (lane < V) store_row_word<T, NT>(&orow[lane], val);
             }
-            if (i < end && lane < V) store_row_word<T, NT>(&out[i * V + lane], ok ? val : T(0));
+            orow += V;
         }
         const uint64_t livemask = ballot(live);
         if (p.err && lane == 0) {
