@@ -33,7 +33,7 @@ HBM_PEAK_GBPS = 8000.0
 # HBM bytes per launch of the plain fetch, measured with two rocprofv3 --pmc
 # passes (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0, factors from a 256 MiB device
 # copy), keyed by (rows, vec_size, batch); profiles/hbm_traffic_fetch_gnn_minibatch.md.
-MEASURED_TRAFFIC = {(2449029, 50, 200000): 50792 * 1024 * 2 + 86885 * 1024}
+MEASURED_TRAFFIC = {(2449029, 50, 200000): 53608 * 1024 * 2 + 78127 * 1024}
 
 
 def log(*a):
