@@ -28,41 +28,80 @@ struct Thresholds {
 template <typename T>
 using WideT = typename std::conditional<sizeof(T) == 8, uint64_t, uint32_t>::type;
 
-// cnt[j*B + k] += rows of [r0, r1) whose column j has bit k set.  Lane <-> column
-// of a 64-column tile (blockIdx.y); the block's waves split the rows; every
-// thread keeps its B counters in registers.
+// Column-bit counts of the row block blockIdx.x and the 64-column tile
+// blockIdx.y: part[blockIdx.x][k*V + j] = rows of the block whose column j has
+// bit k set.  Lane <-> column; the block's waves split the rows; every thread
+// keeps its B counters in registers.  Rows are taken 15 at a time (15 loads in
+// flight) and added into bit-sliced 4-bit counters with a carry-save adder --
+// 7 logic operations per row instead of B extract-and-adds -- which are folded
+// into the B counters once per 15 rows.  The waves' counters are summed in LDS
+// and stored once per block (no global atomics: every block writes its own
+// partial counts, reduce_counts_kernel adds them up).
 template <typename T>
 __global__ __launch_bounds__(256) void count_bits_kernel(const T *data, int64_t num_vecs, int64_t V,
-                                                         int64_t rows_per_block, int32_t *cnt)
+                                                         int64_t rows_per_block, int32_t *part)
 {
     constexpr int B = 8 * (int)sizeof(T);
+    constexpr int CHUNK = 15; // 4-bit counters
+    using W = WideT<T>;
+    __shared__ int32_t sum[B * kWave];
     const int lane = lane_id();
     const int wave_in_block = wave_in_block_id();
     const int waves = blockDim.x / kWave;
     const int64_t j = (int64_t)blockIdx.y * kWave + lane;
+    const bool col = j < V;
     const int64_t r0 = (int64_t)blockIdx.x * rows_per_block;
     int64_t r1 = r0 + rows_per_block;
     if (r1 > num_vecs) r1 = num_vecs;
-    if (j >= V) return;
+    for (int i = threadIdx.x; i < B * kWave; i += blockDim.x) sum[i] = 0;
     int32_t c[B];
 #pragma unroll
     for (int k = 0; k < B; ++k) c[k] = 0;
-    int64_t r = r0 + wave_in_block;
-    // Two rows per iteration keep two loads in flight per lane.
-    for (; r + waves < r1; r += 2 * waves) {
-        const WideT<T> x0 = (WideT<T>)data[r * V + j];
-        const WideT<T> x1 = (WideT<T>)data[(r + waves) * V + j];
+    for (int64_t r = r0 + wave_in_block; r < r1; r += (int64_t)CHUNK * waves) {
+        W x[CHUNK];
 #pragma unroll
-        for (int k = 0; k < B; ++k) c[k] += (int32_t)((x0 >> k) & 1) + (int32_t)((x1 >> k) & 1);
-    }
-    for (; r < r1; r += waves) {
-        const WideT<T> x = (WideT<T>)data[r * V + j];
+        for (int q = 0; q < CHUNK; ++q) {
+            const int64_t rr = r + (int64_t)q * waves;
+            x[q] = (col && rr < r1) ? (W)data[rr * V + j] : W(0);
+        }
+        W ones = 0, twos = 0, fours = 0, eights = 0;
 #pragma unroll
-        for (int k = 0; k < B; ++k) c[k] += (int32_t)((x >> k) & 1);
+        for (int q = 0; q < CHUNK; ++q) {
+            const W c1 = ones & x[q];
+            ones ^= x[q];
+            const W c2 = twos & c1;
+            twos ^= c1;
+            const W c4 = fours & c2;
+            fours ^= c2;
+            eights |= c4; // at most 15 rows: nothing carries out of eights
+        }
+#pragma unroll
+        for (int k = 0; k < B; ++k)
+            c[k] += (int32_t)((ones >> k) & 1) + ((int32_t)((twos >> k) & 1) << 1) +
+                    ((int32_t)((fours >> k) & 1) << 2) + ((int32_t)((eights >> k) & 1) << 3);
     }
+    __syncthreads();
 #pragma unroll
     for (int k = 0; k < B; ++k)
-        if (c[k]) atomicAdd(&cnt[j * B + k], c[k]);
+        if (c[k]) atomicAdd(&sum[k * kWave + lane], c[k]);
+    __syncthreads();
+    int32_t *out = part + (int64_t)blockIdx.x * V * B;
+    for (int i = threadIdx.x; i < B * kWave; i += blockDim.x) {
+        const int64_t jj = (int64_t)blockIdx.y * kWave + (i & (kWave - 1));
+        if (jj < V) out[(int64_t)(i / kWave) * V + jj] = sum[i];
+    }
+}
+
+// cnt[j*B + k] = sum over the row blocks of part[b][k*V + j].
+__global__ void reduce_counts_kernel(const int32_t *part, int64_t blocks, int64_t V, int B, int32_t *cnt)
+{
+    const int64_t n = V * B;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        int32_t acc = 0;
+        for (int64_t b = 0; b < blocks; ++b) acc += part[b * n + i];
+        const int64_t k = i / V, j = i % V;
+        cnt[j * B + k] = acc;
+    }
 }
 
 // masks[t*V + j], bitvals[t*V + j] for every threshold t
@@ -107,37 +146,64 @@ __global__ __launch_bounds__(256) void score_kernel(const T *data, int64_t num_v
     unsigned long long saved[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) saved[t] = 0;
-    // Rows of <= 64 elements: every lane keeps its column's masks in registers.
-    T m_reg[NT], b_reg[NT];
-    int pc_reg[NT];
-    const bool small = V <= kWave;
+    if (V <= kWave) {
+        // Rows of <= 64 elements: every lane keeps its column's masks in
+        // registers; 4 rows in flight per wave; two thresholds share one
+        // 32-bit reduction (a row's sum is at most 64 * 64 < 2^16).
+        constexpr int R = 4;
+        constexpr int NP = (NT + 1) / 2;
+        T m_reg[NT], b_reg[NT];
+        uint32_t pc_reg[NT];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-        const bool ok = small && t < nt && lane < V;
-        m_reg[t] = ok ? masks[t * V + lane] : T(0);
-        b_reg[t] = ok ? bitvals[t * V + lane] : T(0);
-        pc_reg[t] = popc((WideT<T>)m_reg[t]);
-    }
-    for (int64_t r = wave; r < num_vecs; r += nwaves) {
-        int acc[NT];
+        for (int t = 0; t < NT; ++t) {
+            const bool ok = t < nt && lane < V;
+            m_reg[t] = ok ? masks[t * V + lane] : T(0);
+            b_reg[t] = ok ? bitvals[t * V + lane] : T(0);
+            // a lane past the row (or a threshold past nt) never adds anything
+            pc_reg[t] = ok ? (uint32_t)popc((WideT<T>)m_reg[t]) << (16 * (t & 1)) : 0u;
+        }
+        for (int64_t r = wave * R; r < num_vecs; r += nwaves * R) {
+            T x[R];
 #pragma unroll
-        for (int t = 0; t < NT; ++t) acc[t] = 0;
-        for (int64_t j = lane; j < V; j += kWave) {
-            const T x = data[r * V + j];
+            for (int i = 0; i < R; ++i) x[i] = (r + i < num_vecs && lane < V) ? data[(r + i) * V + lane] : T(0);
+#pragma unroll
+            for (int i = 0; i < R; ++i) {
+                if (r + i >= num_vecs) break; // uniform
+                uint32_t acc[NP];
+#pragma unroll
+                for (int q = 0; q < NP; ++q) acc[q] = 0;
+#pragma unroll
+                for (int t = 0; t < NT; ++t)
+                    if ((T)(x[i] & m_reg[t]) == b_reg[t]) acc[t >> 1] += pc_reg[t];
+#pragma unroll
+                for (int q = 0; q < NP; ++q) {
+                    const uint32_t sum = (uint32_t)wave_sum((int)acc[q]);
+                    const int64_t s0 = sum & 0xffffu, s1 = sum >> 16;
+                    if (s0 > V) saved[2 * q] += (unsigned long long)(s0 - V);
+                    if (2 * q + 1 < NT && s1 > V) saved[2 * q + 1] += (unsigned long long)(s1 - V);
+                }
+            }
+        }
+    } else {
+        for (int64_t r = wave; r < num_vecs; r += nwaves) {
+            int acc[NT];
+#pragma unroll
+            for (int t = 0; t < NT; ++t) acc[t] = 0;
+            for (int64_t j = lane; j < V; j += kWave) {
+                const T x = data[r * V + j];
+#pragma unroll
+                for (int t = 0; t < NT; ++t) {
+                    if (t >= nt) break;
+                    const T m = masks[t * V + j];
+                    if ((T)(x & m) == bitvals[t * V + j]) acc[t] += popc((WideT<T>)m);
+                }
+            }
 #pragma unroll
             for (int t = 0; t < NT; ++t) {
                 if (t >= nt) break;
-                const T m = small ? m_reg[t] : masks[t * V + j];
-                const T b = small ? b_reg[t] : bitvals[t * V + j];
-                const int pc = small ? pc_reg[t] : popc((WideT<T>)m);
-                if ((T)(x & m) == b) acc[t] += pc;
+                const int64_t s = wave_sum(acc[t]);
+                if (s > V) saved[t] += (unsigned long long)(s - V);
             }
-        }
-#pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            if (t >= nt) break;
-            const int64_t s = wave_sum(acc[t]);
-            if (s > V) saved[t] += (unsigned long long)(s - V);
         }
     }
     if (lane == 0)
@@ -173,12 +239,6 @@ __global__ void zero_u64_kernel(unsigned long long *p, int n)
     if (i < n) p[i] = 0;
 }
 
-__global__ void zero_i32_kernel(int32_t *p, int64_t n)
-{
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        p[i] = 0;
-}
-
 // The thresholds the reference tries (ibp_preproc_host.cuh:61-67): only
 // `threshold` if it is >= 0, else 0.7f, then t = float(double(t) + 0.05) while
 // t <= 1.0f.
@@ -196,8 +256,27 @@ Thresholds threshold_list(float threshold)
 namespace {
 
 struct PreWorkspace {
-    size_t cnt_off, masks_off, bitvals_off, scores_off, total;
+    size_t cnt_off, masks_off, bitvals_off, scores_off, part_off, total;
 };
+
+// Row blocks of count_bits_kernel: about 1024 blocks in all (four per CU of a
+// 256-CU device), at least one per 64-column tile, at least 64 rows each.
+constexpr int64_t kCountBlocks = 1024;
+int64_t count_row_blocks(int64_t N, int64_t V, int64_t *rows_per_block)
+{
+    const int64_t tiles = (V + kWave - 1) / kWave;
+    int64_t want = kCountBlocks / tiles;
+    if (want < 1) want = 1;
+    int64_t rpb = (N + want - 1) / want;
+    if (rpb < 64) rpb = 64;
+    if (rows_per_block) *rows_per_block = rpb;
+    return N > 0 ? (N + rpb - 1) / rpb : 0;
+}
+int64_t max_count_row_blocks(int64_t V)
+{
+    const int64_t tiles = (V + kWave - 1) / kWave;
+    return kCountBlocks / tiles > 1 ? kCountBlocks / tiles : 1;
+}
 
 PreWorkspace layout(int w, int64_t V, int nt)
 {
@@ -207,7 +286,8 @@ PreWorkspace layout(int w, int64_t V, int nt)
     L.masks_off = al(L.cnt_off + sizeof(int32_t) * (size_t)V * 8 * w);
     L.bitvals_off = al(L.masks_off + (size_t)w * V * nt);
     L.scores_off = al(L.bitvals_off + (size_t)w * V * nt);
-    L.total = al(L.scores_off + sizeof(unsigned long long) * kMaxThresholds);
+    L.part_off = al(L.scores_off + sizeof(unsigned long long) * kMaxThresholds);
+    L.total = al(L.part_off + sizeof(int32_t) * (size_t)max_count_row_blocks(V) * V * 8 * w);
     return L;
 }
 
@@ -221,23 +301,23 @@ int preprocess_typed(const void *data, int64_t N, int64_t V, const Thresholds &t
     T *masks = (T *)(ws + L.masks_off);
     T *bitvals = (T *)(ws + L.bitvals_off);
     unsigned long long *scores = (unsigned long long *)(ws + L.scores_off);
+    int32_t *part = (int32_t *)(ws + L.part_off);
     const int64_t ncnt = V * 8 * (int64_t)sizeof(T);
-    hipLaunchKernelGGL(zero_i32_kernel, dim3((unsigned)((ncnt + 255) / 256 < 1024 ? (ncnt + 255) / 256 : 1024)),
-                       dim3(256), 0, stream, cnt, ncnt);
     hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, stream, scores, kMaxThresholds);
     IBP_LAUNCH_CHECK();
-    if (N > 0) {
-        // Enough row blocks to cover every CU a few times per column tile.
+    int64_t rows_per_block = 0;
+    const int64_t row_blocks = count_row_blocks(N, V, &rows_per_block);
+    if (row_blocks > 0) {
         const int64_t tiles = (V + kWave - 1) / kWave;
-        int64_t want_blocks = (int64_t)dev.num_cus * 4 / tiles;
-        if (want_blocks < 1) want_blocks = 1;
-        int64_t rows_per_block = (N + want_blocks - 1) / want_blocks;
-        if (rows_per_block < 64) rows_per_block = 64;
-        const int64_t row_blocks = (N + rows_per_block - 1) / rows_per_block;
         hipLaunchKernelGGL((count_bits_kernel<T>), dim3((unsigned)row_blocks, (unsigned)tiles), dim3(256), 0, stream,
-                           (const T *)data, N, V, rows_per_block, cnt);
+                           (const T *)data, N, V, rows_per_block, part);
         IBP_LAUNCH_CHECK();
     }
+    int64_t rgrid = (ncnt + 255) / 256;
+    if (rgrid > 4096) rgrid = 4096;
+    hipLaunchKernelGGL(reduce_counts_kernel, dim3((unsigned)rgrid), dim3(256), 0, stream, (const int32_t *)part,
+                       row_blocks, V, 8 * (int)sizeof(T), cnt);
+    IBP_LAUNCH_CHECK();
     const int64_t mthreads = V * th.count;
     hipLaunchKernelGGL((create_masks_kernel<T>), dim3((unsigned)((mthreads + 255) / 256)), dim3(256), 0, stream, cnt, N,
                        V, th, masks, bitvals);
