@@ -18,6 +18,7 @@
 #include <type_traits>
 
 #include "ibp_bits.h"
+#include "ibp_fetch_dev.h"
 #include "ibp_internal.h"
 #include "ibp_wave.h"
 
@@ -61,18 +62,57 @@ __global__ __launch_bounds__(256) void size_kernel(const T *data, int64_t num_ve
     const int64_t wave = global_wave_id();
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
     unsigned long long acc = 0, bad = 0;
-    for (int64_t i = wave; i < n; i += nwaves) {
-        const int64_t r = readfirstlane(row_of<IdxT>(index, i));
-        int64_t size = 0;
-        const int64_t c = (r >= 0 && r < num_vecs) ? mask_row_of(clusters, num_clusters, r) : -1;
-        if (c >= 0) {
-            const int saved = row_saved_bits<T>(data + r * V, mask + c * V, bitval + c * V, V);
-            size = slot_size(V, (int)sizeof(T), saved);
-        } else {
-            ++bad;
+    if (V <= kWave && clusters == nullptr) {
+        // Rows of <= 64 elements with one mask: the lane's column constants
+        // stay in registers and R rows are in flight per wave.
+        constexpr int R = 4;
+        using W = typename std::conditional<sizeof(T) == 8, uint64_t, uint32_t>::type;
+        const bool col = lane < V;
+        const T m = col ? mask[lane] : T(0), b = col ? bitval[lane] : T(1);
+        const int pc = col ? popc((W)m) : 0; // lanes past the row never match (b has a bit outside m)
+        for (int64_t i0 = wave * R; i0 < n; i0 += nwaves * R) {
+            int64_t r[R];
+            T x[R];
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                r[q] = i0 + q < n ? readfirstlane(row_of<IdxT>(index, i0 + q)) : -1;
+                const bool ok = r[q] >= 0 && r[q] < num_vecs;
+                x[q] = (ok && col) ? data[r[q] * V + lane] : T(0);
+            }
+            // Two rows per 32-bit reduction: a row saves at most 64 * 64 < 2^16 bits.
+            int saved[R];
+#pragma unroll
+            for (int q = 0; q < R; q += 2) {
+                const int s2 = wave_sum(((T)(x[q] & m) == b ? pc : 0) | (((T)(x[q + 1] & m) == b ? pc : 0) << 16));
+                saved[q] = s2 & 0xffff;
+                saved[q + 1] = s2 >> 16;
+            }
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                if (i0 + q >= n) break; // uniform
+                int64_t size = 0;
+                if (r[q] >= 0 && r[q] < num_vecs)
+                    size = slot_size(V, (int)sizeof(T), saved[q]);
+                else
+                    ++bad;
+                if (lane == 0) sizes[i0 + q] = size;
+                acc += (unsigned long long)size;
+            }
         }
-        if (lane == 0) sizes[i] = size;
-        acc += (unsigned long long)size;
+    } else {
+        for (int64_t i = wave; i < n; i += nwaves) {
+            const int64_t r = readfirstlane(row_of<IdxT>(index, i));
+            int64_t size = 0;
+            const int64_t c = (r >= 0 && r < num_vecs) ? mask_row_of(clusters, num_clusters, r) : -1;
+            if (c >= 0) {
+                const int saved = row_saved_bits<T>(data + r * V, mask + c * V, bitval + c * V, V);
+                size = slot_size(V, (int)sizeof(T), saved);
+            } else {
+                ++bad;
+            }
+            if (lane == 0) sizes[i] = size;
+            acc += (unsigned long long)size;
+        }
     }
     if (lane == 0) {
         if (total && acc) atomicAdd(total, acc);
@@ -104,6 +144,19 @@ __device__ __forceinline__ void or_word(T *buf, int64_t q, T v)
         uint32_t *d = (uint32_t *)((unsigned char *)buf + (byte & ~int64_t(3)));
         __hip_atomic_fetch_or(d, (uint32_t)v << (8 * (int)(byte & 3)), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
+}
+
+// Free bits of a flagged element gathered MSB first (the inverse of
+// small_deposit): run k of the column's free bits moves up by shift[k].
+template <typename T>
+__device__ __forceinline__ T small_extract(const SmallCols<T> &c, T x)
+{
+    if (c.wave_runs > kRunCap) return extract_msb<T>(x, c.free);
+    T v = T(0);
+#pragma unroll
+    for (int k = 0; k < kRunCap; ++k)
+        if (k < c.wave_runs) v |= T(T(x & c.run[k]) << c.shift[k]);
+    return v;
 }
 
 // Bytes of one wave's slot buffer: V words padded to 16 bytes.
@@ -152,6 +205,12 @@ __global__ __launch_bounds__(256) void pack_kernel(PackParams p)
     const bool in_place = !CONDENSED && p.in == (const void *)p.out && p.index == nullptr;
     const int64_t ngroups = (p.n + 31) / 32;
     unsigned long long bad = 0;
+    // Rows of <= 64 elements with one mask: the lane's column constants and
+    // run table stay in registers, and each row is read once.
+    const bool small = V <= kWave && p.clusters == nullptr;
+    SmallCols<T> col{};
+    if (small) col = load_small_cols<T>(p.mask, p.bitval, (int)V);
+    const int len_raw = col.valid ? B : 0;
 
     for (int64_t g = wave; g < ngroups; g += nwaves) {
         uint32_t bits = 0;
@@ -166,8 +225,17 @@ __global__ __launch_bounds__(256) void pack_kernel(PackParams p)
             const T *src = in + r * V;
             const T *mask = (const T *)p.mask + c * V;
             const T *bitval = (const T *)p.bitval + c * V;
+            T xs = T(0);
+            bool fs = false;
+            int saved;
+            if (small) {
+                xs = col.valid ? src[lane] : T(0);
+                fs = col.valid && (T)(xs & T(~col.free)) == col.bitval;
+                saved = wave_sum(fs ? B - col.len_flagged : 0);
+            } else {
+                saved = row_saved_bits<T>(src, mask, bitval, V);
+            }
             // Size test (ibp_compress_kernel.cuh:28-31): pack only if smaller.
-            const int saved = row_saved_bits<T>(src, mask, bitval, V);
             const int64_t size = slot_size(V, W, saved);
             T *dst;
             if constexpr (CONDENSED) {
@@ -184,14 +252,35 @@ __global__ __launch_bounds__(256) void pack_kernel(PackParams p)
                 dst = (T *)p.out + i * V;
             }
             if (size >= vec_bytes) {
-                if (!in_place)
-                    for (int64_t k = lane; k < V; k += kWave) dst[k] = src[k];
+                if (!in_place) {
+                    if (small) {
+                        if (col.valid) dst[lane] = xs;
+                    } else {
+                        for (int64_t k = lane; k < V; k += kWave) dst[k] = src[k];
+                    }
+                }
                 continue;
             }
             for (int64_t k = lane; k < buf_bytes / 4; k += kWave) ((uint32_t *)bufb)[k] = 0u;
             slot_fence<BUF_LDS>();
             int64_t cursor = 0;
-            for (int64_t c0 = 0; c0 < V; c0 += kWave) {
+            if (small) {
+                const uint64_t hb = ballot(fs);
+                if (lane < kWave / B && lane * B < V) {
+                    const T hw = (T)(hb >> (lane * B));
+                    if (hw) or_word<T>(buf, lane, hw);
+                }
+                const int L = fs ? col.len_flagged : len_raw;
+                const unsigned pbit = (unsigned)(wave_incl_scan(L) - L);
+                const T v = fs ? small_extract<T>(col, xs) : xs;
+                const int64_t q = Hw + pbit / B;
+                const int o = (int)(pbit % B);
+                if (L > 0) {
+                    or_word<T>(buf, q, shr<T>(v, o));
+                    if (o + L > B) or_word<T>(buf, q + 1, shl<T>(v, B - o));
+                }
+            }
+            for (int64_t c0 = 0; !small && c0 < V; c0 += kWave) {
                 const int64_t j = c0 + lane;
                 const bool valid = j < V;
                 const T x = valid ? src[j] : T(0);
