@@ -492,6 +492,37 @@ def run_serve(args):
             "server_stats": {k: (round(v, 2) if isinstance(v, float) else v) for k, v in st.items()}}
 
 
+def run_gnn_features(args):
+    """GNN features (65536 x 128 fp32 as int64 [65536 x 64], N(0, 0.1) with 30%
+    zeros; preprocess at 0.9): one decompress_fetch of 4096 random rows per
+    call -- a latency-bound call.  Reported: per-call wall latency (issue ->
+    rows ready) from HBM and from pinned host memory, and the GPU time of a
+    call (HIP events over back-to-back calls)."""
+    import invariantbitpacking_amd as ibp
+    from invariantbitpacking_amd import workloads
+    x, idx = workloads.gnn_features()
+    xd = x.cuda()
+    mask, bitval = ibp.preprocess(xd, 0.9)
+    comp = xd.clone()
+    bm = ibp.compress_inplace(comp, mask, bitval)
+    cl = ibp.get_single_compress_len(xd, mask, bitval)
+    pinned = comp.cpu().pin_memory()
+    idx_d = idx.cuda()
+    out = torch.empty((idx.numel(), x.shape[1]), dtype=torch.int64, device="cuda")
+    want = xd[idx_d]
+    res = {"workload": "gnn_features: 65536 x 64 int64 (fp32 [65536 x 128]), 4096 random rows per call",
+           "out_bytes_per_call": out.numel() * 8}
+    for where, src in (("hbm", comp), ("pinned", pinned)):
+        f = lambda: ibp.decompress_fetch(src, mask, bitval, bm, torch.device("cuda"), cl, idx_d, None, None, None, out)
+        gpu_ms, _ = timed_region(f, args.steps, args.warmup)
+        lat = call_latencies(f, max(50, args.steps))
+        res[where] = {"p50_ms": round(float(np.percentile(lat, 50)), 4), "p99_ms": round(float(np.percentile(lat, 99)), 4),
+                      "gpu_ms_per_call": round(gpu_ms, 4), "gbps_decompressed": round(out.numel() * 8 / gpu_ms / 1e6, 1),
+                      "bitexact": bool(torch.equal(out, want))}
+        log("gnn_features %s: %s" % (where, res[where]))
+    return res
+
+
 def dec_bytes(nbytes, ms):
     """GB/s of nbytes moved in ms milliseconds."""
     return nbytes / (ms / 1e3) / 1e9
@@ -615,7 +646,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-pcie", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
-    ap.add_argument("--workload", default="gnn_minibatch", choices=["gnn_minibatch", "dlrm", "serve", "llm"],
+    ap.add_argument("--workload", default="gnn_minibatch", choices=["gnn_minibatch", "gnn_features", "dlrm", "serve", "llm"],
                     help="gnn_minibatch is the headline line; the others print their own JSON")
     ap.add_argument("--rates", type=float, nargs="+", default=[1e2, 1e3, 1e4])
     ap.add_argument("--layers", type=int, default=2, help="decoder layers of the llm workload")
@@ -637,6 +668,8 @@ def main():
         print(json.dumps(run_serve(args)))
     elif args.workload == "llm":
         print(json.dumps(run_llm(args)))
+    elif args.workload == "gnn_features":
+        print(json.dumps(run_gnn_features(args)))
     else:
         print(json.dumps(run_gnn_minibatch(args)))
 

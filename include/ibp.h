@@ -235,6 +235,11 @@ typedef struct ibp_server_config {
 } ibp_server_config;
 
 int ibp_server_create(const ibp_server_config *cfg, ibp_server **srv);
+/* Stops the server: launches in flight are completed, pending requests are
+ * dropped, all memory is freed.  No other call on the same server may be in
+ * progress or follow.  The other ibp_server_* calls are thread-safe with
+ * respect to each other (one submitting thread and one waiting thread is the
+ * usual pattern); a request must be waited for and released exactly once. */
 int ibp_server_destroy(ibp_server *srv);
 /* Registers a compressed tensor set; returns its id in *set_id. */
 int ibp_server_register(ibp_server *srv, int elem_bytes, const void *comp, int64_t num_vecs,

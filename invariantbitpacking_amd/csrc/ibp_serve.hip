@@ -389,14 +389,15 @@ extern "C" int ibp_server_wait(ibp_server *srv, int64_t req_id, int timeout_ms, 
     std::unique_lock<std::mutex> lk(srv->mu);
     auto it = srv->reqs.find(req_id);
     IBP_CHECK(it != srv->reqs.end(), "unknown request %lld", (long long)req_id);
-    auto ready = [&] { return srv->reqs[req_id].st == St::Done || !srv->worker_error.empty(); };
+    // std::map iterators stay valid while other requests come and go.
+    auto ready = [&] { return it->second.st == St::Done || !srv->worker_error.empty(); };
     if (timeout_ms < 0) {
         srv->cv_done.wait(lk, ready);
     } else if (!srv->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready)) {
         return set_error(IBP_ERR_BUSY, "request %lld not finished", (long long)req_id);
     }
     if (!srv->worker_error.empty()) return set_error(IBP_ERR_HIP, "server worker: %s", srv->worker_error.c_str());
-    const Req &r = srv->reqs[req_id];
+    const Req &r = it->second;
     if (out) *out = srv->arena_mem + r.out_off;
     if (latency_us) *latency_us = r.t_done - r.t_arrival;
     return IBP_OK;
