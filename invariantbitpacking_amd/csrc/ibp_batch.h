@@ -9,6 +9,10 @@
 
 namespace ibp {
 
+// Kinds of task in a launch (bit flags): rows of <= 64 elements of 8 or of 4
+// bytes, and wider rows.
+enum : int { kTasksSmall8 = 1, kTasksSmall4 = 2, kTasksLarge = 4, kTasksAll = 7 };
+
 struct TaskDev {
     FetchParams p;
     int64_t item_begin;
@@ -16,6 +20,8 @@ struct TaskDev {
     int32_t index_bytes;
     int32_t rows_per_item;
     int32_t small;
+    int32_t block_begin; // segmented launch: the task's first block
+    int32_t blocks;      // segmented launch: the task's blocks
 };
 
 struct BatchConfig {
@@ -25,6 +31,19 @@ struct BatchConfig {
     int blocks;              // persistent grid (256 threads per block)
     int claim;               // work items claimed per queue atomic (dynamic mode)
     int dynamic;             // 1: atomic work queue, 0: static item striding
+    int segmented;           // 1: a segmented launch where the tasks allow it
+};
+
+// What fill_tasks found: the kinds of task present and, when the launch is
+// segmented (every task has rows of <= 64 elements of one element width and
+// one index width), the grid.  A segmented launch gives each task its own
+// range of blocks, in which the task runs exactly the plain fetch's loop; it
+// needs no work queue and the registers of the plain kernel.
+struct BatchShape {
+    int kinds = kTasksAll;
+    int seg_blocks = 0;  // 0: persistent launch over work items
+    int elem_bytes = 8;  // segmented launch
+    int index_bytes = 8; // segmented launch (8 also for tasks without index)
 };
 
 // A launch shape for `total_rows` rows: work items small enough that every
@@ -33,12 +52,14 @@ struct BatchConfig {
 BatchConfig adapt_config(const BatchConfig &base, int64_t total_rows, int num_cus);
 
 // Fills `out[0..n)` from the public task descriptions; returns the number of
-// work items.
-int64_t fill_tasks(const ibp_fetch_task *tasks, int n, const BatchConfig &cfg, TaskDev *out);
+// work items and fills *shape.  `max_blocks` caps a segmented grid.
+int64_t fill_tasks(const ibp_fetch_task *tasks, int n, const BatchConfig &cfg, int max_blocks, TaskDev *out,
+                   BatchShape *shape);
 
-// Launches the persistent kernel on `s` (dynamic mode: after a one-thread
-// kernel that resets the queue head).
+// Launches the fetch of the tasks on `s`: the segmented kernel when
+// shape.seg_blocks > 0, else the persistent kernel compiled for shape.kinds
+// (dynamic mode: after a one-thread kernel that resets the queue head).
 int launch_batch(const TaskDev *dev_tasks, int ntasks, int64_t total_items, unsigned long long *head,
-                 const BatchConfig &cfg, hipStream_t s);
+                 const BatchConfig &cfg, hipStream_t s, const BatchShape &shape);
 
 } // namespace ibp

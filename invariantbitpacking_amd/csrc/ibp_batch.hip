@@ -26,33 +26,41 @@ __global__ void batch_reset_kernel(unsigned long long *head)
     if (threadIdx.x == 0 && blockIdx.x == 0) *head = 0;
 }
 
-// Rows [b, e) of task t.  Column tables of small-row tasks are cached across
-// consecutive items of the same task (cached_task).
-template <int ROWS>
-__device__ __forceinline__ void run_item(const TaskDev &t, int task, int64_t b, int64_t e, int &cached_task,
-                                         SmallCols<uint64_t> &c64, SmallCols<uint32_t> &c32)
+// Rows [b, e) of task t.  KINDS (kTasks* flags) says which kinds of task the
+// launch holds: a kernel compiled for one kind needs far fewer registers than
+// one holding every path, so more waves fit.  A small-row task's column table
+// is built per item (two loads per lane); keeping it across items would hold
+// registers for the whole loop.
+template <int ROWS, int KINDS>
+__device__ __forceinline__ void run_item(const TaskDev &t, int64_t b, int64_t e)
 {
     const FetchParams &p = t.p;
     const bool i32 = p.index != nullptr && t.index_bytes == 4;
-    if (t.small) {
-        if (t.elem_bytes == 8) {
-            if (cached_task != task) c64 = load_small_cols<uint64_t>(p.mask, p.bitval, (int)p.vec_size);
-            if (i32) fetch_small_range<uint64_t, int32_t, ROWS>(p, c64, b, e, ROWS);
-            else fetch_small_range<uint64_t, int64_t, ROWS>(p, c64, b, e, ROWS);
-        } else {
-            if (cached_task != task) c32 = load_small_cols<uint32_t>(p.mask, p.bitval, (int)p.vec_size);
-            if (i32) fetch_small_range<uint32_t, int32_t, ROWS>(p, c32, b, e, ROWS);
-            else fetch_small_range<uint32_t, int64_t, ROWS>(p, c32, b, e, ROWS);
+    const int kind = !t.small ? kTasksLarge : t.elem_bytes == 8 ? kTasksSmall8 : kTasksSmall4;
+    if constexpr ((KINDS & kTasksLarge) != 0) {
+        if (KINDS == kTasksLarge || kind == kTasksLarge) {
+            if (t.elem_bytes == 8) {
+                if (i32) fetch_large_range<uint64_t, int32_t>(p, b, e, 1);
+                else fetch_large_range<uint64_t, int64_t>(p, b, e, 1);
+            } else {
+                if (i32) fetch_large_range<uint32_t, int32_t>(p, b, e, 1);
+                else fetch_large_range<uint32_t, int64_t>(p, b, e, 1);
+            }
+            return;
         }
-        cached_task = task;
-    } else {
-        if (t.elem_bytes == 8) {
-            if (i32) fetch_large_range<uint64_t, int32_t>(p, b, e, 1);
-            else fetch_large_range<uint64_t, int64_t>(p, b, e, 1);
-        } else {
-            if (i32) fetch_large_range<uint32_t, int32_t>(p, b, e, 1);
-            else fetch_large_range<uint32_t, int64_t>(p, b, e, 1);
+    }
+    if constexpr ((KINDS & kTasksSmall8) != 0) {
+        if (KINDS == kTasksSmall8 || kind == kTasksSmall8) {
+            const SmallCols<uint64_t> c = load_small_cols<uint64_t>(p.mask, p.bitval, (int)p.vec_size);
+            if (i32) fetch_small_range<uint64_t, int32_t, ROWS>(p, c, b, e, ROWS);
+            else fetch_small_range<uint64_t, int64_t, ROWS>(p, c, b, e, ROWS);
+            return;
         }
+    }
+    if constexpr ((KINDS & kTasksSmall4) != 0) {
+        const SmallCols<uint32_t> c = load_small_cols<uint32_t>(p.mask, p.bitval, (int)p.vec_size);
+        if (i32) fetch_small_range<uint32_t, int32_t, ROWS>(p, c, b, e, ROWS);
+        else fetch_small_range<uint32_t, int64_t, ROWS>(p, c, b, e, ROWS);
     }
 }
 
@@ -65,8 +73,8 @@ constexpr int kLdsTasks = 1024;
 // atomic on the queue head until it is drained (uneven item costs).  The
 // item -> task lookup is a binary search over first-item offsets staged in
 // LDS, skipped while consecutive items stay in the current task.
-template <int ROWS>
-__global__ __launch_bounds__(256) void batch_kernel(const TaskDev *tasks, int ntasks, int64_t total_items,
+template <int ROWS, int KINDS>
+__global__ __launch_bounds__(256) void batch_kernel(const TaskDev *__restrict__ tasks, int ntasks, int64_t total_items,
                                                     unsigned long long *head, int claim, int dynamic)
 {
     __shared__ int64_t first_item[kLdsTasks];
@@ -75,12 +83,9 @@ __global__ __launch_bounds__(256) void batch_kernel(const TaskDev *tasks, int nt
     if (staged)
         for (int k = threadIdx.x; k < ntasks; k += blockDim.x) first_item[k] = tasks[k].item_begin;
     __syncthreads();
-    int cached_task = -1;
     int cur = -1;
     int64_t cur_lo = 0, cur_hi = 0; // items [cur_lo, cur_hi) belong to task `cur`
     TaskDev t;
-    SmallCols<uint64_t> c64;
-    SmallCols<uint32_t> c32;
     const int64_t wave = global_wave_id();
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
     int64_t next = wave; // static mode: next item of this wave
@@ -107,17 +112,42 @@ __global__ __launch_bounds__(256) void batch_kernel(const TaskDev *tasks, int nt
                     if (fm <= item) lo = mid;
                     else hi = mid - 1;
                 }
-                cur = lo;
-                t = tasks[lo];
+                // The search ran on LDS values: make the task index visibly
+                // uniform so the descriptor is a scalar load held in SGPRs.
+                cur = readfirstlane(lo);
+                t = tasks[cur];
                 cur_lo = t.item_begin;
-                cur_hi = lo + 1 < ntasks ? (staged ? first_item[lo + 1] : tasks[lo + 1].item_begin) : total_items;
+                cur_hi = cur + 1 < ntasks ? (staged ? first_item[cur + 1] : tasks[cur + 1].item_begin) : total_items;
+                cur_hi = readfirstlane(cur_hi);
             }
             const int64_t b = (item - t.item_begin) * t.rows_per_item;
             int64_t e = b + t.rows_per_item;
             if (e > t.p.n) e = t.p.n;
-            run_item<ROWS>(t, cur, b, e, cached_task, c64, c32);
+            run_item<ROWS, KINDS>(t, b, e);
         }
     }
+}
+
+// Segmented launch: task t owns blocks [block_begin, block_begin + blocks)
+// and runs the plain fetch's loop over its rows on that sub-grid.  The block ->
+// task search runs on blockIdx, so it and the task's parameters are scalar.
+template <typename T, typename IdxT, int ROWS>
+__global__ __launch_bounds__(256) void batch_segmented_kernel(const TaskDev *__restrict__ tasks, int ntasks)
+{
+    const int blk = (int)blockIdx.x;
+    int lo = 0, hi = ntasks - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (tasks[mid].block_begin <= blk) lo = mid;
+        else hi = mid - 1;
+    }
+    const TaskDev &t = tasks[lo];
+    const FetchParams &p = t.p;
+    const SmallCols<T> col = load_small_cols<T>(p.mask, p.bitval, (int)p.vec_size);
+    const int64_t waves_per_block = blockDim.x / kWave;
+    const int64_t wave = (int64_t)(blk - t.block_begin) * waves_per_block + wave_in_block_id();
+    const int64_t nwaves = (int64_t)t.blocks * waves_per_block;
+    fetch_small_range<T, IdxT, ROWS>(p, col, wave * ROWS, p.n, nwaves * ROWS);
 }
 
 BatchConfig adapt_config(const BatchConfig &base, int64_t total_rows, int num_cus)
@@ -133,9 +163,14 @@ BatchConfig adapt_config(const BatchConfig &base, int64_t total_rows, int num_cu
     return c;
 }
 
-int64_t fill_tasks(const ibp_fetch_task *tasks, int n, const BatchConfig &cfg, TaskDev *out)
+// Rows in flight of a segmented launch (4 or 8).
+static int seg_rows(const BatchConfig &cfg) { return cfg.rows_in_flight >= 8 ? 8 : 4; }
+
+int64_t fill_tasks(const ibp_fetch_task *tasks, int n, const BatchConfig &cfg, int max_blocks, TaskDev *out,
+                   BatchShape *shape)
 {
-    int64_t items = 0;
+    int64_t items = 0, rows = 0;
+    int kinds = 0, idx_mask = 0;
     for (int i = 0; i < n; ++i) {
         const ibp_fetch_task &t = tasks[i];
         TaskDev d;
@@ -154,17 +189,94 @@ int64_t fill_tasks(const ibp_fetch_task *tasks, int n, const BatchConfig &cfg, T
         d.elem_bytes = t.elem_bytes;
         d.index_bytes = t.index_bytes;
         d.small = t.vec_size <= kWave;
+        kinds |= !d.small ? kTasksLarge : t.elem_bytes == 8 ? kTasksSmall8 : kTasksSmall4;
+        idx_mask |= (t.index != nullptr && t.index_bytes == 4) ? 1 : 2;
         d.rows_per_item = d.small ? cfg.rows_per_item_small : cfg.rows_per_item_large;
         d.item_begin = items;
         items += (t.n + d.rows_per_item - 1) / d.rows_per_item;
+        rows += t.n;
         out[i] = d;
     }
+    BatchShape sh;
+    sh.kinds = kinds ? kinds : kTasksAll;
+    if (cfg.segmented && (kinds == kTasksSmall8 || kinds == kTasksSmall4) && (idx_mask == 1 || idx_mask == 2) &&
+        rows > 0) {
+        // Blocks per task: one group of rows per wave, as the plain fetch
+        // launches, shrunk in proportion to the task's rows when the total
+        // exceeds max_blocks (each task with rows keeps at least one block).
+        const int64_t per_block = (int64_t)seg_rows(cfg) * 4;
+        int64_t natural = 0;
+        for (int i = 0; i < n; ++i) natural += (out[i].p.n + per_block - 1) / per_block;
+        const int64_t cap = max_blocks > 0 ? max_blocks : (int64_t)1 << 22;
+        int64_t begin = 0;
+        for (int i = 0; i < n; ++i) {
+            int64_t b = (out[i].p.n + per_block - 1) / per_block;
+            if (natural > cap && b > 0) {
+                b = (int64_t)((double)cap * (double)out[i].p.n / (double)rows);
+                if (b < 1) b = 1;
+            }
+            out[i].block_begin = (int32_t)begin;
+            out[i].blocks = (int32_t)b;
+            begin += b;
+        }
+        sh.seg_blocks = (int)begin;
+        sh.elem_bytes = kinds == kTasksSmall8 ? 8 : 4;
+        sh.index_bytes = idx_mask == 1 ? 4 : 8;
+    }
+    if (shape) *shape = sh;
     return items;
 }
 
-int launch_batch(const TaskDev *dev_tasks, int ntasks, int64_t total_items, unsigned long long *head,
-                 const BatchConfig &cfg, hipStream_t s)
+template <typename T, typename IdxT>
+void launch_segmented(int rows, int blocks, hipStream_t s, const TaskDev *dev_tasks, int ntasks)
 {
+    if (rows == 8)
+        hipLaunchKernelGGL((batch_segmented_kernel<T, IdxT, 8>), dim3(blocks), dim3(256), 0, s, dev_tasks, ntasks);
+    else
+        hipLaunchKernelGGL((batch_segmented_kernel<T, IdxT, 4>), dim3(blocks), dim3(256), 0, s, dev_tasks, ntasks);
+}
+
+template <int ROWS>
+void launch_rows(int classes, dim3 grid, dim3 block, hipStream_t s, const TaskDev *dev_tasks, int ntasks,
+                 int64_t total_items, unsigned long long *head, const BatchConfig &cfg)
+{
+    // One kind of task (the usual plan): the kernel compiled for that kind.
+    switch (classes) {
+    case kTasksSmall8:
+        hipLaunchKernelGGL((batch_kernel<ROWS, kTasksSmall8>), grid, block, 0, s, dev_tasks, ntasks, total_items, head,
+                           cfg.claim, cfg.dynamic);
+        break;
+    case kTasksSmall4:
+        hipLaunchKernelGGL((batch_kernel<ROWS, kTasksSmall4>), grid, block, 0, s, dev_tasks, ntasks, total_items, head,
+                           cfg.claim, cfg.dynamic);
+        break;
+    case kTasksLarge:
+        hipLaunchKernelGGL((batch_kernel<ROWS, kTasksLarge>), grid, block, 0, s, dev_tasks, ntasks, total_items, head,
+                           cfg.claim, cfg.dynamic);
+        break;
+    default:
+        hipLaunchKernelGGL((batch_kernel<ROWS, kTasksAll>), grid, block, 0, s, dev_tasks, ntasks, total_items, head,
+                           cfg.claim, cfg.dynamic);
+        break;
+    }
+}
+
+int launch_batch(const TaskDev *dev_tasks, int ntasks, int64_t total_items, unsigned long long *head,
+                 const BatchConfig &cfg, hipStream_t s, const BatchShape &shape)
+{
+    if (shape.seg_blocks > 0) {
+        const int rows = seg_rows(cfg), b = shape.seg_blocks;
+        if (shape.elem_bytes == 8) {
+            if (shape.index_bytes == 4) launch_segmented<uint64_t, int32_t>(rows, b, s, dev_tasks, ntasks);
+            else launch_segmented<uint64_t, int64_t>(rows, b, s, dev_tasks, ntasks);
+        } else {
+            if (shape.index_bytes == 4) launch_segmented<uint32_t, int32_t>(rows, b, s, dev_tasks, ntasks);
+            else launch_segmented<uint32_t, int64_t>(rows, b, s, dev_tasks, ntasks);
+        }
+        IBP_LAUNCH_CHECK();
+        return IBP_OK;
+    }
+    const int classes = shape.kinds;
     // No more blocks than there is work for (claims of `claim` items in dynamic
     // mode, one item per wave in static mode; four waves per block).
     const int64_t per_wave = cfg.dynamic ? cfg.claim : 1;
@@ -174,9 +286,9 @@ int launch_batch(const TaskDev *dev_tasks, int ntasks, int64_t total_items, unsi
     const dim3 grid((unsigned)blocks), block(256);
     if (cfg.dynamic) hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, s, head);
     switch (cfg.rows_in_flight) {
-    case 2: hipLaunchKernelGGL((batch_kernel<2>), grid, block, 0, s, dev_tasks, ntasks, total_items, head, cfg.claim, cfg.dynamic); break;
-    case 8: hipLaunchKernelGGL((batch_kernel<8>), grid, block, 0, s, dev_tasks, ntasks, total_items, head, cfg.claim, cfg.dynamic); break;
-    default: hipLaunchKernelGGL((batch_kernel<4>), grid, block, 0, s, dev_tasks, ntasks, total_items, head, cfg.claim, cfg.dynamic); break;
+    case 2: launch_rows<2>(classes, grid, block, s, dev_tasks, ntasks, total_items, head, cfg); break;
+    case 8: launch_rows<8>(classes, grid, block, s, dev_tasks, ntasks, total_items, head, cfg); break;
+    default: launch_rows<4>(classes, grid, block, s, dev_tasks, ntasks, total_items, head, cfg); break;
     }
     IBP_LAUNCH_CHECK();
     return IBP_OK;
@@ -192,7 +304,8 @@ struct ibp_plan {
     TaskDev *dev_tasks = nullptr;
     unsigned long long *head = nullptr;
     int64_t total_items = 0;
-    BatchConfig cfg{128, 2, 8, 0, 4, 0};
+    BatchShape shape;
+    BatchConfig cfg{128, 2, 8, 0, 4, 0, 1};
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     hipStream_t capture_stream = nullptr;
@@ -204,7 +317,8 @@ namespace {
 int rebuild(ibp_plan *pl)
 {
     pl->host_tasks.resize(pl->tasks.size());
-    pl->total_items = fill_tasks(pl->tasks.data(), (int)pl->tasks.size(), pl->cfg, pl->host_tasks.data());
+    pl->total_items = fill_tasks(pl->tasks.data(), (int)pl->tasks.size(), pl->cfg, pl->cfg.blocks * 2,
+                                 pl->host_tasks.data(), &pl->shape);
     IBP_HIP(hipMemcpy(pl->dev_tasks, pl->host_tasks.data(), sizeof(TaskDev) * pl->host_tasks.size(),
                       hipMemcpyHostToDevice));
     if (pl->graph_exec) {
@@ -220,7 +334,8 @@ int rebuild(ibp_plan *pl)
 
 int launch(ibp_plan *pl, hipStream_t s)
 {
-    return launch_batch(pl->dev_tasks, (int)pl->host_tasks.size(), pl->total_items, pl->head, pl->cfg, s);
+    return launch_batch(pl->dev_tasks, (int)pl->host_tasks.size(), pl->total_items, pl->head, pl->cfg, s,
+                        pl->shape);
 }
 
 } // namespace
@@ -280,12 +395,14 @@ extern "C" int ibp_plan_autotune(ibp_plan *pl, void *stream, float *best_ms)
     hipStream_t s = as_stream(stream);
     const DeviceInfo &dev = device_info();
     std::vector<BatchConfig> cands;
+    for (int rif : {4, 8})
+        for (int mult : {4, 8, 16}) cands.push_back(BatchConfig{128, 2, rif, dev.num_cus * mult, 1, 0, 1});
     for (int rpi : {32, 64, 128, 256})
         for (int rif : {4, 8})
             for (int mult : {8, 16}) {
-                cands.push_back(BatchConfig{rpi, rpi >= 128 ? 4 : 2, rif, dev.num_cus * mult, 1, 0});
+                cands.push_back(BatchConfig{rpi, rpi >= 128 ? 4 : 2, rif, dev.num_cus * mult, 1, 0, 0});
                 for (int claim : {1, 8})
-                    cands.push_back(BatchConfig{rpi, rpi >= 128 ? 4 : 2, rif, dev.num_cus * mult, claim, 1});
+                    cands.push_back(BatchConfig{rpi, rpi >= 128 ? 4 : 2, rif, dev.num_cus * mult, claim, 1, 0});
             }
     hipEvent_t e0, e1;
     IBP_HIP(hipEventCreate(&e0));
@@ -318,8 +435,9 @@ extern "C" int ibp_plan_autotune(ibp_plan *pl, void *stream, float *best_ms)
     const int st = rebuild(pl);
     if (st != IBP_OK) return st;
     if (best_ms) *best_ms = best;
-    dprintf("[ibp] autotune: rows/item %d/%d, rows in flight %d, blocks %d: %.4f ms\n", best_cfg.rows_per_item_small,
-            best_cfg.rows_per_item_large, best_cfg.rows_in_flight, best_cfg.blocks, best);
+    dprintf("[ibp] autotune: rows/item %d/%d, rows in flight %d, blocks %d, segmented %d: %.4f ms\n",
+            best_cfg.rows_per_item_small, best_cfg.rows_per_item_large, best_cfg.rows_in_flight, best_cfg.blocks,
+            pl->shape.seg_blocks, best);
     return IBP_OK;
 }
 
@@ -356,8 +474,10 @@ extern "C" int ibp_plan_describe(ibp_plan *pl, char *buf, size_t len)
     IBP_CHECK(pl != nullptr && buf != nullptr, "null argument");
     snprintf(buf, len,
              "{\"tasks\": %zu, \"items\": %lld, \"rows_per_item_small\": %d, \"rows_per_item_large\": %d, "
-             "\"rows_in_flight\": %d, \"blocks\": %d, \"claim\": %d, \"dynamic\": %d, \"graph\": %s}",
+             "\"rows_in_flight\": %d, \"blocks\": %d, \"claim\": %d, \"dynamic\": %d, \"segmented_blocks\": %d, "
+             "\"graph\": %s}",
              pl->tasks.size(), (long long)pl->total_items, pl->cfg.rows_per_item_small, pl->cfg.rows_per_item_large,
-             pl->cfg.rows_in_flight, pl->cfg.blocks, pl->cfg.claim, pl->cfg.dynamic, pl->graph_exec ? "true" : "false");
+             pl->cfg.rows_in_flight, pl->cfg.blocks, pl->cfg.claim, pl->cfg.dynamic, pl->shape.seg_blocks,
+             pl->graph_exec ? "true" : "false");
     return IBP_OK;
 }

@@ -130,7 +130,7 @@ struct ibp_server {
     char *pin_mem = nullptr; // pinned host memory of request indices
     Arena pin_arena;
     size_t budget_override = 0;
-    BatchConfig bcfg{256, 4, 8, 0, 1, 0};
+    BatchConfig bcfg{256, 4, 8, 0, 1, 0, 0};
     std::vector<Set> sets;
     std::vector<Slot> slots;
     std::unordered_map<int64_t, Req> reqs;
@@ -209,10 +209,11 @@ struct ibp_server {
         }
         if (sl.reqs.empty()) return false;
         const BatchConfig cfg_now = adapt_config(bcfg, nrows, device_info().num_cus);
-        const int64_t items = fill_tasks(tasks.data(), (int)tasks.size(), cfg_now, sl.host_tasks);
+        BatchShape shape;
+        const int64_t items = fill_tasks(tasks.data(), (int)tasks.size(), cfg_now, 0, sl.host_tasks, &shape);
         const hipError_t e = hipMemcpyAsync(sl.dev_tasks, sl.host_tasks, sizeof(TaskDev) * tasks.size(),
                                             hipMemcpyHostToDevice, sl.stream);
-        int st = e == hipSuccess ? launch_batch(sl.dev_tasks, (int)tasks.size(), items, sl.head, cfg_now, sl.stream)
+        int st = e == hipSuccess ? launch_batch(sl.dev_tasks, (int)tasks.size(), items, sl.head, cfg_now, sl.stream, shape)
                                  : IBP_ERR_HIP;
         if (st == IBP_OK && hipEventRecord(sl.done, sl.stream) != hipSuccess) st = IBP_ERR_HIP;
         if (st != IBP_OK) worker_error = "launch failed";
