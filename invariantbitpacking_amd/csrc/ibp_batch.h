@@ -56,6 +56,10 @@ BatchConfig adapt_config(const BatchConfig &base, int64_t total_rows, int num_cu
 int64_t fill_tasks(const ibp_fetch_task *tasks, int n, const BatchConfig &cfg, int max_blocks, TaskDev *out,
                    BatchShape *shape);
 
+// Launches the one-thread kernel that zeroes the work-queue head on `s` (also
+// used to make the runtime set up a stream's hardware queue ahead of time).
+int reset_queue_head(unsigned long long *head, hipStream_t s);
+
 // Launches the fetch of the tasks on `s`: the segmented kernel when
 // shape.seg_blocks > 0, else the persistent kernel compiled for shape.kinds
 // (dynamic mode: after a one-thread kernel that resets the queue head).

@@ -184,7 +184,8 @@ int64_t fill_tasks(const ibp_fetch_task *tasks, int n, const BatchConfig &cfg, i
         d.p.num_vecs = t.num_vecs;
         d.p.vec_size = t.vec_size;
         d.p.n = t.n;
-        d.p.first_words = first_words_for(t.vec_size, t.elem_bytes, t.comp_len);
+        const bool host = t.comp_loc == IBP_LOC_HOST || (t.comp_loc == IBP_LOC_AUTO && pointer_is_host(t.comp));
+        d.p.first_words = first_words_for(t.vec_size, t.elem_bytes, t.comp_len, host);
         d.p.err = t.index ? index_error_counter() : nullptr;
         d.elem_bytes = t.elem_bytes;
         d.index_bytes = t.index_bytes;
@@ -259,6 +260,13 @@ void launch_rows(int classes, dim3 grid, dim3 block, hipStream_t s, const TaskDe
                            cfg.claim, cfg.dynamic);
         break;
     }
+}
+
+int reset_queue_head(unsigned long long *head, hipStream_t s)
+{
+    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, s, head);
+    IBP_LAUNCH_CHECK();
+    return IBP_OK;
 }
 
 int launch_batch(const TaskDev *dev_tasks, int ntasks, int64_t total_items, unsigned long long *head,

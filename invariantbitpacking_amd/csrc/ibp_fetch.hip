@@ -48,18 +48,24 @@ int fetch_launch(const FetchParams &p, int elem_bytes, int index_bytes, int impl
 }
 
 // Words of the speculative first read of a packed small row: the expected
-// compressed length plus three words of slack, at least the header.  Rows
-// scatter a word or two around the average length; a row longer than the
-// first read costs its group a second memory round trip, which on MI355X
-// costs more than the few extra bytes (GNN minibatch: 38.2 us with one word
-// of slack, 36.0 us with three).
+// compressed length plus some slack, at least the header.  Rows scatter a
+// word or two around the average length; a row longer than the first read
+// costs its group a second memory round trip.  From HBM that round trip costs
+// more than a few extra bytes (GNN minibatch: 38.2 us with one word of slack,
+// 36.0 us with three).  Over PCIe, where reads go out in 128-byte lines and
+// bytes are the bound, one word of slack is best (the same rows from pinned
+// memory: 1.760 ms with one word, 1.813 ms with three).
 #ifndef IBP_FIRST_READ_SLACK
 #define IBP_FIRST_READ_SLACK 3
 #endif
-int first_words_for(int64_t vec_size, int elem_bytes, int comp_len)
+#ifndef IBP_FIRST_READ_SLACK_HOST
+#define IBP_FIRST_READ_SLACK_HOST 1
+#endif
+int first_words_for(int64_t vec_size, int elem_bytes, int comp_len, bool host_resident)
 {
     const int64_t hw = header_bytes(vec_size, elem_bytes) / elem_bytes;
-    int64_t e = comp_len > 0 ? (int64_t)comp_len + IBP_FIRST_READ_SLACK : vec_size;
+    const int slack = host_resident ? IBP_FIRST_READ_SLACK_HOST : IBP_FIRST_READ_SLACK;
+    int64_t e = comp_len > 0 ? (int64_t)comp_len + slack : vec_size;
     if (e < hw + 1) e = hw + 1;
     if (e > vec_size) e = vec_size;
     return (int)e;
@@ -102,7 +108,8 @@ extern "C" int ibp_decompress_fetch_ex(int elem_bytes, void *out, const void *co
     p.num_vecs = num_vecs;
     p.vec_size = vec_size;
     p.n = n;
-    p.first_words = first_words_for(vec_size, elem_bytes, comp_len);
+    const bool host = comp_loc == IBP_LOC_HOST || (comp_loc == IBP_LOC_AUTO && pointer_is_host(comp));
+    p.first_words = first_words_for(vec_size, elem_bytes, comp_len, host);
     p.pad = 0;
     p.err = (index || condensed || clustered) ? index_error_counter() : nullptr;
     int lay = kLayoutSlots;
@@ -117,7 +124,6 @@ extern "C" int ibp_decompress_fetch_ex(int elem_bytes, void *out, const void *co
         p.num_clusters = layout->num_clusters;
     }
     const int cl = comp_len > 0 ? comp_len : (int)vec_size;
-    const bool host = comp_loc == IBP_LOC_HOST || (comp_loc == IBP_LOC_AUTO && pointer_is_host(comp));
     return fetch_launch(p, elem_bytes, index_bytes, impl, cl, nblks, nthreads, as_stream(stream), lay, host);
 }
 

@@ -35,7 +35,7 @@ bool pointer_is_host(const void *p);
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
 // Words of the speculative first read of a packed row of <= 64 elements.
-int first_words_for(int64_t vec_size, int elem_bytes, int comp_len);
+int first_words_for(int64_t vec_size, int elem_bytes, int comp_len, bool host_resident);
 
 } // namespace ibp
 

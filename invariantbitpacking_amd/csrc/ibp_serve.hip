@@ -26,6 +26,12 @@
 
 using namespace ibp;
 
+// 1: launches whose requests all have rows of <= 64 elements of one width run
+// segmented (each request its own blocks, the plain fetch's loop).
+#ifndef IBP_SERVE_SEGMENTED
+#define IBP_SERVE_SEGMENTED 1
+#endif
+
 namespace {
 
 using Clock = std::chrono::steady_clock;
@@ -130,7 +136,7 @@ struct ibp_server {
     char *pin_mem = nullptr; // pinned host memory of request indices
     Arena pin_arena;
     size_t budget_override = 0;
-    BatchConfig bcfg{256, 4, 8, 0, 1, 0, 0};
+    BatchConfig bcfg{256, 4, 8, 0, 1, 0, IBP_SERVE_SEGMENTED};
     std::vector<Set> sets;
     std::vector<Slot> slots;
     std::unordered_map<int64_t, Req> reqs;
@@ -210,7 +216,8 @@ struct ibp_server {
         if (sl.reqs.empty()) return false;
         const BatchConfig cfg_now = adapt_config(bcfg, nrows, device_info().num_cus);
         BatchShape shape;
-        const int64_t items = fill_tasks(tasks.data(), (int)tasks.size(), cfg_now, 0, sl.host_tasks, &shape);
+        const int64_t items =
+            fill_tasks(tasks.data(), (int)tasks.size(), cfg_now, device_info().num_cus * 16, sl.host_tasks, &shape);
         const hipError_t e = hipMemcpyAsync(sl.dev_tasks, sl.host_tasks, sizeof(TaskDev) * tasks.size(),
                                             hipMemcpyHostToDevice, sl.stream);
         int st = e == hipSuccess ? launch_batch(sl.dev_tasks, (int)tasks.size(), items, sl.head, cfg_now, sl.stream, shape)
@@ -304,6 +311,14 @@ extern "C" int ibp_server_create(const ibp_server_config *cfg, ibp_server **out)
         if ((e = hipMalloc((void **)&sl.dev_tasks, stage)) != hipSuccess) return fail("task table", e);
         if ((e = hipHostMalloc((void **)&sl.host_tasks, stage, 0)) != hipSuccess) return fail("task staging", e);
         if ((e = hipMalloc((void **)&sl.head, sizeof(unsigned long long))) != hipSuccess) return fail("queue", e);
+        // The runtime sets up a stream's hardware queue at its first launch
+        // (tens of ms): do that here, not in the first request that lands on it.
+        const int st = reset_queue_head(sl.head, sl.stream);
+        if (st != IBP_OK || (e = hipStreamSynchronize(sl.stream)) != hipSuccess) {
+            const std::string why = st != IBP_OK ? ibp_last_error() : hipGetErrorString(e);
+            ibp_server_destroy(srv);
+            return set_error(IBP_ERR_HIP, "server stream warm-up failed: %s", why.c_str());
+        }
     }
     srv->worker = std::thread([srv] { srv->run(); });
     *out = srv;
@@ -341,7 +356,9 @@ extern "C" int ibp_server_register(ibp_server *srv, int elem_bytes, const void *
     IBP_CHECK(elem_bytes == 4 || elem_bytes == 8, "element size must be 4 or 8");
     IBP_CHECK(comp && mask && bitval && bitmask && vec_size > 0 && num_vecs >= 0, "invalid tensor set");
     std::lock_guard<std::mutex> lk(srv->mu);
-    srv->sets.push_back(Set{elem_bytes, comp, num_vecs, vec_size, mask, bitval, bitmask, comp_len, comp_loc});
+    // resolved once here: the worker fills task descriptors per launch
+    const int loc = comp_loc == IBP_LOC_AUTO ? (pointer_is_host(comp) ? IBP_LOC_HOST : IBP_LOC_DEVICE) : comp_loc;
+    srv->sets.push_back(Set{elem_bytes, comp, num_vecs, vec_size, mask, bitval, bitmask, comp_len, loc});
     *set_id = (int)srv->sets.size() - 1;
     return IBP_OK;
 }

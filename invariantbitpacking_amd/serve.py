@@ -50,13 +50,16 @@ class Server:
         _capi.call("ibp_server_submit", self._srv, set_id, idx.ctypes.data, idx.size, ctypes.byref(rid))
         return rid.value, set_id, idx.size
 
-    def wait(self, req, timeout_ms=-1):
+    def wait(self, req, timeout_ms=-1, view=True):
         """Returns (view of the [n x V] result in the arena, latency in us).  The
-        view stays valid until release(req)."""
+        view stays valid until release(req).  view=False returns (None, latency)
+        without building the tensor."""
         rid, set_id, n = req
         out = ctypes.c_void_p()
         lat = ctypes.c_double(0)
         _capi.call("ibp_server_wait", self._srv, rid, int(timeout_ms), ctypes.byref(out), ctypes.byref(lat))
+        if not view:
+            return None, lat.value
         v, dtype = self._sets[set_id]
         typestr = {torch.int64: "<i8", torch.float64: "<f8", torch.int32: "<i4", torch.float32: "<f4"}[dtype]
         t = torch.as_tensor(_ArenaView(out.value, (n, v), typestr), device="cuda")
