@@ -557,6 +557,44 @@ def run_serve(args):
             "server_stats": {k: (round(v, 2) if isinstance(v, float) else v) for k, v in st.items()}}
 
 
+def run_pcie_rows(args):
+    """What random rows from pinned memory can reach over PCIe, by row size:
+    uncompressed int64 rows of 400, 512 and 1024 bytes (about 1 GB per table)
+    gathered by decompress_fetch's raw-row path (impl 2, all-zero bitmask), 200,000
+    uniform random rows per call, against a pinned 256 MiB H2D copy.  Reads
+    from host memory go out in 128-byte lines: rows that are a whole number of
+    aligned lines reach the copy rate, a 400-byte row at a 400-byte stride
+    pays for 4-5 lines."""
+    import invariantbitpacking_amd as ibp
+    ceiling = h2d_ceiling_gbps()
+    n = 200000
+    g = torch.Generator().manual_seed(7)
+    res = {"workload": "pcie_rows: 200000 random uncompressed rows from pinned memory per call",
+           "h2d_copy_ceiling_gbps": round(ceiling, 2), "rows": []}
+    for row_bytes in (400, 512, 1024):
+        v = row_bytes // 8
+        num = (1 << 30) // row_bytes
+        table = torch.randint(-1 << 62, 1 << 62, (num, v), dtype=torch.int64, generator=g).pin_memory()
+        zero = torch.zeros((num + 31) // 32, dtype=torch.int32, device="cuda")
+        mask = torch.zeros(v, dtype=torch.int64, device="cuda")
+        idx = [torch.randint(0, num, (n,), generator=g).cuda() for _ in range(4)]
+        out = torch.empty((n, v), dtype=torch.int64, device="cuda")
+        step = [0]
+
+        def gather():
+            ibp.decompress_fetch(table, mask, mask, zero, torch.device("cuda"), v, idx[step[0] % 4], None, None, 2,
+                                 out)
+            step[0] += 1
+        ms, _ = timed_region(gather, 8, 2)
+        exact = bool(torch.equal(out.cpu(), table[idx[(step[0] - 1) % 4].cpu()]))
+        gbps = n * row_bytes / (ms / 1e3) / 1e9
+        res["rows"].append({"row_bytes": row_bytes, "ms_per_call": round(ms, 4), "gbps": round(gbps, 2),
+                            "frac_of_ceiling": round(gbps / ceiling, 3), "bitexact": exact})
+        log("pcie rows %d B: %s" % (row_bytes, res["rows"][-1]))
+        del table
+    return res
+
+
 def run_gnn_features(args):
     """GNN features (65536 x 128 fp32 as int64 [65536 x 64], N(0, 0.1) with 30%
     zeros; preprocess at 0.9): one decompress_fetch of 4096 random rows per
@@ -711,7 +749,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-pcie", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
-    ap.add_argument("--workload", default="gnn_minibatch", choices=["gnn_minibatch", "gnn_features", "dlrm", "serve", "llm"],
+    ap.add_argument("--workload", default="gnn_minibatch",
+                    choices=["gnn_minibatch", "gnn_features", "dlrm", "serve", "llm", "pcie_rows"],
                     help="gnn_minibatch is the headline line; the others print their own JSON")
     ap.add_argument("--rates", type=float, nargs="+", default=[1e2, 1e3, 1e4])
     ap.add_argument("--layers", type=int, default=2, help="decoder layers of the llm workload")
@@ -735,6 +774,8 @@ def main():
         print(json.dumps(run_llm(args)))
     elif args.workload == "gnn_features":
         print(json.dumps(run_gnn_features(args)))
+    elif args.workload == "pcie_rows":
+        print(json.dumps(run_pcie_rows(args)))
     else:
         print(json.dumps(run_gnn_minibatch(args)))
 

@@ -208,7 +208,9 @@ typedef struct ibp_plan ibp_plan;
 
 /* Builds a plan for a fixed set of tasks (pointers are captured: refill the
  * index/output buffers in place between launches).  The plan owns a device work
- * queue; all tasks run in ONE persistent kernel launch. */
+ * queue; all tasks run in ONE kernel launch: segmented (each task its own
+ * blocks) when every task has rows of <= 64 elements of one element width and
+ * one index width, else persistent over a work queue. */
 int ibp_plan_create(const ibp_fetch_task *tasks, int num_tasks, ibp_plan **plan);
 int ibp_plan_destroy(ibp_plan *plan);
 /* Launch on a stream (graph-capturable: no allocation, no memset node). */
