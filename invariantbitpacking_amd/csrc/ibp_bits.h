@@ -116,4 +116,31 @@ __host__ __device__ __forceinline__ int64_t slot_size(int64_t vec_size, int w, i
     return comp < vec_bytes ? comp : vec_bytes;
 }
 
+
+// Column-bit counting over up to 15 words per call: the words are added into
+// bit-sliced 4-bit counters (ones, twos, fours, eights) with a carry-save
+// adder -- 7 logic operations per word instead of B extract-and-adds -- and
+// the slices are folded into the B per-bit counters once.  W is uint32_t or
+// uint64_t; bits above B are ignored.
+template <typename W, int B, int CHUNK>
+__device__ __forceinline__ void csa_count(const W (&x)[CHUNK], int32_t (&c)[B])
+{
+    static_assert(CHUNK <= 15, "4-bit counters hold at most 15 words");
+    W ones = 0, twos = 0, fours = 0, eights = 0;
+#pragma unroll
+    for (int q = 0; q < CHUNK; ++q) {
+        const W c1 = ones & x[q];
+        ones ^= x[q];
+        const W c2 = twos & c1;
+        twos ^= c1;
+        const W c4 = fours & c2;
+        fours ^= c2;
+        eights |= c4; // at most 15 words: nothing carries out of eights
+    }
+#pragma unroll
+    for (int k = 0; k < B; ++k)
+        c[k] += (int32_t)((ones >> k) & 1) + ((int32_t)((twos >> k) & 1) << 1) + ((int32_t)((fours >> k) & 1) << 2) +
+                ((int32_t)((eights >> k) & 1) << 3);
+}
+
 } // namespace ibp

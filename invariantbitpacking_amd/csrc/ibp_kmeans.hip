@@ -43,6 +43,10 @@ __global__ void km_gather_kernel(const T *data, int64_t V, const int64_t *rows, 
 }
 
 // Nearest centroid by popc(x ^ c); equal distances go to the highest index.
+// One wave per row; each element is loaded once and compared with up to 16
+// centroids (16 distance accumulators per lane), then the 16 distances are
+// summed over the wave.
+constexpr int kAssignGroup = 16;
 template <typename T>
 __global__ __launch_bounds__(256) void km_assign_kernel(const T *data, int64_t N, int64_t V, const T *centroids,
                                                         int K, int32_t *cluster)
@@ -53,16 +57,61 @@ __global__ __launch_bounds__(256) void km_assign_kernel(const T *data, int64_t N
     for (int64_t i = wave; i < N; i += nwaves) {
         int best = 0;
         int best_d = 0x7fffffff;
-        for (int c = 0; c < K; ++c) {
-            int d = 0;
-            for (int64_t j = lane; j < V; j += kWave) d += popc((KW<T>)(data[i * V + j] ^ centroids[(int64_t)c * V + j]));
-            d = wave_sum(d);
-            if (d <= best_d) {
-                best_d = d;
-                best = c;
+        for (int c0 = 0; c0 < K; c0 += kAssignGroup) {
+            int d[kAssignGroup];
+#pragma unroll
+            for (int g = 0; g < kAssignGroup; ++g) d[g] = 0;
+            for (int64_t j = lane; j < V; j += kWave) {
+                const KW<T> x = (KW<T>)data[i * V + j];
+#pragma unroll
+                for (int g = 0; g < kAssignGroup; ++g)
+                    if (c0 + g < K) d[g] += popc((KW<T>)(x ^ (KW<T>)centroids[(int64_t)(c0 + g) * V + j]));
+            }
+#pragma unroll
+            for (int g = 0; g < kAssignGroup; ++g) {
+                if (c0 + g >= K) break; // uniform
+                const int dist = wave_sum(d[g]);
+                if (dist <= best_d) {
+                    best_d = dist;
+                    best = c0 + g;
+                }
             }
         }
         if (lane == 0) cluster[i] = best;
+    }
+}
+
+// Rows of each cluster, grouped (a counting sort of the assignment) in one
+// block of 1024 threads with the histogram in LDS: count[c], offset[c] and
+// perm[offset[c] ..) = the rows of cluster c (in no particular order).  For
+// K <= kSortMaxK; larger K use the three-kernel form below.
+constexpr int kSortMaxK = 4096;
+__global__ __launch_bounds__(1024) void km_sort_kernel(const int32_t *cluster, int64_t N, int K, int32_t *count,
+                                                       int32_t *offset, int32_t *perm)
+{
+    __shared__ int32_t hist[kSortMaxK];
+    __shared__ int32_t base[kSortMaxK];
+    for (int c = threadIdx.x; c < K; c += blockDim.x) hist[c] = 0;
+    __syncthreads();
+    for (int64_t i = threadIdx.x; i < N; i += blockDim.x) atomicAdd(&hist[cluster[i]], 1);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int32_t acc = 0;
+        for (int c = 0; c < K; ++c) {
+            base[c] = acc;
+            acc += hist[c];
+        }
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < K; c += blockDim.x) {
+        count[c] = hist[c];
+        offset[c] = base[c];
+        hist[c] = 0; // becomes the fill counter
+    }
+    __syncthreads();
+    for (int64_t i = threadIdx.x; i < N; i += blockDim.x) {
+        const int c = cluster[i];
+        perm[base[c] + atomicAdd(&hist[c], 1)] = (int32_t)i;
     }
 }
 
@@ -93,51 +142,105 @@ __global__ void km_scatter_kernel(const int32_t *cluster, int64_t N, const int32
 }
 
 // bits_set[(c*V + j)*B + k] = members of cluster c whose column j has bit k set.
-// grid = (column tiles, K, row splits); lane <-> column, waves split the rows.
+// grid = (column tiles, K, row splits); lane <-> column, the block's waves
+// split the cluster's rows, 15 rows at a time (15 gathered loads in flight)
+// added with a carry-save adder (csa_count).  The waves' counters are summed
+// in LDS; with one row split the block owns its counters and stores them,
+// otherwise it adds them to counters zeroed beforehand.
 template <typename T>
 __global__ __launch_bounds__(256) void km_bitcount_kernel(const T *data, int64_t V, const int32_t *perm,
                                                           const int32_t *offset, const int32_t *count,
                                                           int32_t *bits_set)
 {
     constexpr int B = 8 * (int)sizeof(T);
+    constexpr int CHUNK = 15;
+    using W = KW<T>;
+    __shared__ int32_t sum[B * kWave];
     const int lane = lane_id();
+    const int wv = wave_in_block_id();
     const int c = blockIdx.y;
     const int64_t j = (int64_t)blockIdx.x * kWave + lane;
-    if (j >= V) return;
+    const bool col = j < V;
     const int waves = blockDim.x / kWave;
     const int64_t stride = (int64_t)waves * gridDim.z;
     const int32_t *rows = perm + offset[c];
     const int64_t nrows = count[c];
+    for (int i = threadIdx.x; i < B * kWave; i += blockDim.x) sum[i] = 0;
     int32_t acc[B];
 #pragma unroll
     for (int k = 0; k < B; ++k) acc[k] = 0;
-    for (int64_t q = (int64_t)blockIdx.z * waves + threadIdx.x / kWave; q < nrows; q += stride) {
-        const KW<T> x = (KW<T>)data[(int64_t)rows[q] * V + j];
+    for (int64_t q = (int64_t)blockIdx.z * waves + wv; q < nrows; q += (int64_t)CHUNK * stride) {
+        W x[CHUNK];
 #pragma unroll
-        for (int k = 0; k < B; ++k) acc[k] += (int32_t)((x >> k) & 1);
+        for (int u = 0; u < CHUNK; ++u) {
+            const int64_t qi = q + (int64_t)u * stride;
+            x[u] = (col && qi < nrows) ? (W)data[(int64_t)rows[qi] * V + j] : W(0);
+        }
+        csa_count<W, B, CHUNK>(x, acc);
     }
-    int32_t *dst = bits_set + ((int64_t)c * V + j) * B;
+    __syncthreads();
 #pragma unroll
     for (int k = 0; k < B; ++k)
-        if (acc[k]) atomicAdd(&dst[k], acc[k]);
+        if (acc[k]) atomicAdd(&sum[k * kWave + lane], acc[k]);
+    __syncthreads();
+    // thread i -> column tile*64 + i / B, bit i % B: consecutive threads write
+    // consecutive counters
+    for (int i = threadIdx.x; i < B * kWave; i += blockDim.x) {
+        const int jl = i / B, k = i % B;
+        const int64_t jj = (int64_t)blockIdx.x * kWave + jl;
+        if (jj >= V) continue;
+        int32_t *dst = bits_set + ((int64_t)c * V + jj) * B + k;
+        const int32_t v = sum[k * kWave + jl];
+        if (gridDim.z == 1) *dst = v;
+        else if (v) atomicAdd(dst, v);
+    }
+}
+
+// Majority word of cluster c, column j: bit k set iff more than half of the
+// cluster's members have it (the per-column part of construct_bitmasks).
+template <typename T>
+__global__ void km_majority_kernel(int64_t V, int K, const int32_t *count, const int32_t *bits_set, T *maj)
+{
+    constexpr int B = 8 * (int)sizeof(T);
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (int64_t)K * V) return;
+    const int32_t half = count[g / V] / 2;
+    const int32_t *bs = bits_set + g * B;
+    uint64_t acc = 0;
+#pragma unroll
+    for (int k = 0; k < B; ++k)
+        if (bs[k] > half) acc |= 1ull << k;
+    maj[g] = (T)acc;
 }
 
 // construct_bitmasks (ibp_preproc_kmeans.cuh:219-236): thread (c, l), l < 32,
-// walks columns l, l+32, ... and ORs majority bits into one accumulator.
+// walks columns l, l+32, ... and ORs majority bits into one accumulator that
+// is never cleared, so centroid[c][l + 32i] = OR of maj[c][l + 32i'], i' <= i.
+// The majority words come from km_majority_kernel; the chain's loads are
+// issued 8 at a time.
 template <typename T>
-__global__ void km_construct_kernel(int64_t V, int K, const int32_t *count, const int32_t *bits_set, T *centroids)
+__global__ void km_construct_kernel(int64_t V, int K, const T *maj, T *centroids)
 {
-    constexpr int B = 8 * (int)sizeof(T);
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= K * 32) return;
     const int c = g / 32, l = g % 32;
-    const int32_t half = count[c] / 2;
-    uint64_t acc = 0;
-    for (int64_t id = l; id < V; id += 32) {
-        const int32_t *bs = bits_set + ((int64_t)c * V + id) * B;
-        for (int k = 0; k < B; ++k)
-            if (bs[k] > half) acc |= 1ull << k;
-        centroids[(int64_t)c * V + id] = (T)acc;
+    const T *src = maj + (int64_t)c * V;
+    T *dst = centroids + (int64_t)c * V;
+    T acc = 0;
+    int64_t id = l;
+    for (; id + 32 * 7 < V; id += 32 * 8) {
+        T w[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) w[u] = src[id + 32 * u];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            acc = (T)(acc | w[u]);
+            dst[id + 32 * u] = acc;
+        }
+    }
+    for (; id < V; id += 32) {
+        acc = (T)(acc | src[id]);
+        dst[id] = acc;
     }
 }
 
@@ -219,12 +322,13 @@ int kmeans_typed(const void *data_v, int64_t N, int64_t V, int K, int64_t seed, 
         libstdcxx_random_shuffle(idx);
         for (int64_t c = 1; c < (K < N ? K : N); ++c) init[(size_t)c] = idx[(size_t)c];
     }
-    T *cent = nullptr;
+    T *cent = nullptr, *maj = nullptr;
     int32_t *cluster = nullptr, *count = nullptr, *offset = nullptr, *fill = nullptr, *perm = nullptr, *bits = nullptr;
     int64_t *rows = nullptr;
     const int64_t nbits = (int64_t)K * V * B;
     auto cleanup = [&]() {
         (void)hipFreeAsync(cent, s);
+        (void)hipFreeAsync(maj, s);
         (void)hipFreeAsync(cluster, s);
         (void)hipFreeAsync(count, s);
         (void)hipFreeAsync(perm, s);
@@ -233,6 +337,7 @@ int kmeans_typed(const void *data_v, int64_t N, int64_t V, int K, int64_t seed, 
     };
     hipError_t e = hipSuccess;
     if ((e = hipMallocAsync((void **)&cent, sizeof(T) * (size_t)(K * V), s)) != hipSuccess ||
+        (e = hipMallocAsync((void **)&maj, sizeof(T) * (size_t)(K * V), s)) != hipSuccess ||
         (e = hipMallocAsync((void **)&cluster, sizeof(int32_t) * (size_t)(N > 0 ? N : 1), s)) != hipSuccess ||
         (e = hipMallocAsync((void **)&count, sizeof(int32_t) * 3 * (size_t)K, s)) != hipSuccess ||
         (e = hipMallocAsync((void **)&perm, sizeof(int32_t) * (size_t)(N > 0 ? N : 1), s)) != hipSuccess ||
@@ -252,20 +357,33 @@ int kmeans_typed(const void *data_v, int64_t N, int64_t V, int K, int64_t seed, 
     if (assign_grid > (int64_t)dev.num_cus * 8) assign_grid = (int64_t)dev.num_cus * 8;
     if (assign_grid < 1) assign_grid = 1;
     const unsigned tiles = (unsigned)((V + kWave - 1) / kWave);
-    unsigned splits = (unsigned)((N / (K > 0 ? K : 1)) / 1024 + 1);
-    if (splits > 64) splits = 64;
+    // Row splits of the bit count: about 64 rows of a cluster per wave (four
+    // waves per block); one split lets each block store its counters.
+    const int64_t per_cluster = N / (K > 0 ? K : 1);
+    int64_t splits64 = (per_cluster + 255) / 256;
+    if (splits64 < 1) splits64 = 1;
+    if (splits64 > 64) splits64 = 64;
+    const unsigned splits = (unsigned)splits64;
+    const bool sort_in_lds = K <= kSortMaxK && N <= ((int64_t)1 << 20);
     for (int it = 0; it < 10; ++it) {
         hipLaunchKernelGGL((km_assign_kernel<T>), dim3((unsigned)assign_grid), dim3(256), 0, s, data, N, V, cent, K,
                            cluster);
-        hipLaunchKernelGGL(km_zero_kernel, dim3(1), dim3(256), 0, s, count, (int64_t)K);
-        hipLaunchKernelGGL(km_zero_kernel, dim3(blocks_for(nbits)), dim3(256), 0, s, bits, nbits);
-        hipLaunchKernelGGL(km_hist_kernel, dim3(blocks_for(N)), dim3(256), 0, s, cluster, N, count);
-        hipLaunchKernelGGL(km_offsets_kernel, dim3(1), dim3(64), 0, s, count, K, offset, fill);
-        hipLaunchKernelGGL(km_scatter_kernel, dim3(blocks_for(N)), dim3(256), 0, s, cluster, N, offset, fill, perm);
+        if (sort_in_lds) {
+            hipLaunchKernelGGL(km_sort_kernel, dim3(1), dim3(1024), 0, s, cluster, N, K, count, offset, perm);
+        } else {
+            hipLaunchKernelGGL(km_zero_kernel, dim3(1), dim3(256), 0, s, count, (int64_t)K);
+            hipLaunchKernelGGL(km_hist_kernel, dim3(blocks_for(N)), dim3(256), 0, s, cluster, N, count);
+            hipLaunchKernelGGL(km_offsets_kernel, dim3(1), dim3(64), 0, s, count, K, offset, fill);
+            hipLaunchKernelGGL(km_scatter_kernel, dim3(blocks_for(N)), dim3(256), 0, s, cluster, N, offset, fill,
+                               perm);
+        }
+        if (splits > 1) hipLaunchKernelGGL(km_zero_kernel, dim3(blocks_for(nbits)), dim3(256), 0, s, bits, nbits);
         hipLaunchKernelGGL((km_bitcount_kernel<T>), dim3(tiles, (unsigned)K, splits), dim3(256), 0, s, data, V, perm,
                            offset, count, bits);
+        hipLaunchKernelGGL((km_majority_kernel<T>), dim3(blocks_for((int64_t)K * V, 1 << 20)), dim3(256), 0, s, V, K,
+                           count, bits, maj);
         hipLaunchKernelGGL((km_construct_kernel<T>), dim3((unsigned)((K * 32 + 255) / 256)), dim3(256), 0, s, V, K,
-                           count, bits, cent);
+                           (const T *)maj, cent);
         e = hipGetLastError();
         if (e != hipSuccess) {
             cleanup();

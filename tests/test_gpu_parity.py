@@ -69,6 +69,47 @@ def test_preprocess_golden(ibp, oracle):
         assert [hex(x) for x in _u(_np(b))] == [hex(int(x, 16)) for x in c["bitval"]], c["name"]
 
 
+def banded_rows(n, v, w, seed):
+    """Rows whose bit k of column j takes the column's majority value in a
+    fraction of the rows drawn from [0.6, 1.0]: column-bit counts fall on both
+    sides of every sweep threshold, so every threshold saves a different number
+    of bits and the choice among them depends on each score."""
+    rng = np.random.default_rng(seed)
+    B = 8 * w
+    frac = rng.uniform(0.6, 1.0, size=(v, B))
+    majority = rng.random((v, B)) < 0.5
+    words = np.zeros((n, v), dtype=np.uint64)
+    for k in range(B):
+        bit = (rng.random((n, v)) < frac[:, k]) == majority[:, k]
+        words |= bit.astype(np.uint64) << np.uint64(k)
+    return words.astype(np.dtype("u%d" % w)).view(np.dtype("i%d" % w))
+
+
+@pytest.mark.parametrize("threshold", [None, 0.9, 0.3])
+@pytest.mark.parametrize("shape", [(3000, 64, 8), (2000, 50, 8), (1500, 48, 4), (777, 128, 4), (500, 130, 8),
+                                   (4096, 2, 8), (100, 4, 4), (900, 66, 4)])
+def test_preprocess_choice_matches_oracle(ibp, oracle, threshold, shape):
+    """The chosen sweep step (C ABI output) and the masks, on data whose
+    thresholds all score differently; covers the row-per-lane score path (rows
+    of whole 16-byte chunks) and the lane-per-column one (900 x 66 x 4 B)."""
+    from invariantbitpacking_amd import _capi
+    n, v, w = shape
+    thr = -1.0 if threshold is None else threshold
+    for data in (banded_rows(n, v, w, seed=n + v), float_rows(n, v, w, seed=v + w)):
+        print("preprocess choice", shape, threshold, flush=True)
+        t = place(data, "device")
+        mask = torch.zeros(v, dtype=t.dtype, device="cuda")
+        bitval = torch.zeros(v, dtype=t.dtype, device="cuda")
+        step = torch.full((1,), -7, dtype=torch.int32, device="cuda")
+        ws_bytes = _capi.lib().ibp_preprocess_workspace_size(w, v, thr)
+        ws = torch.empty(ws_bytes, dtype=torch.uint8, device="cuda")
+        _capi.call("ibp_preprocess", w, t.data_ptr(), n, v, thr, mask.data_ptr(), bitval.data_ptr(), step.data_ptr(),
+                   ws.data_ptr(), ws_bytes, torch.cuda.current_stream().cuda_stream)
+        om, ob, ostep, _ = oracle.preprocess(data, threshold)
+        assert int(step[0]) == ostep
+        assert np.array_equal(_u(_np(mask)), om) and np.array_equal(_u(_np(bitval)), ob)
+
+
 # ------------------------------------------------------------------ size + pack
 
 SHAPES = [(200, 1), (200, 3), (500, 50), (700, 64), (300, 65), (200, 130), (64, 1024), (40, 2752)]
@@ -263,8 +304,14 @@ def test_fetch_full_width_fields(ibp, oracle):
 
 # ------------------------------------------------------------------ kmeans
 
-@pytest.mark.parametrize("w,n,v,k", [(4, 60, 40, 4), (8, 60, 70, 3), (8, 300, 64, 16), (4, 10, 5, 16)])
+@pytest.mark.parametrize("w,n,v,k", [(4, 60, 40, 4), (8, 60, 70, 3), (8, 300, 64, 16), (4, 10, 5, 16),
+                                     (8, 4096, 130, 16), (4, 3000, 300, 20), (8, 6000, 64, 4),
+                                     (4, 1100000, 2, 3)])
 def test_kmeans_matches_oracle(ibp, oracle, w, n, v, k):
+    """Small shapes; 4096 x 130 (one block per cluster and column tile);
+    K = 20 (two groups of 16 centroids in the assignment); 6000 rows in 4
+    clusters (row splits adding into zeroed counters); 1.1M rows (the counting
+    sort over several kernels instead of one block)."""
     rng = np.random.default_rng(n + v + k)
     dt = np.dtype("u%d" % w)
     base = rng.integers(0, 1 << 62, size=(4, v), dtype=np.uint64).astype(dt)
