@@ -129,9 +129,14 @@ __global__ void create_masks_kernel(const int32_t *cnt, int64_t num_vecs, int64_
 {
     constexpr int B = 8 * (int)sizeof(T);
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= V * th.count) return;
+    if (gid >= V * kMaxThresholds) return;
     const int t = (int)(gid / V);
     const int64_t j = gid % V;
+    if (t >= th.count) { // unused thresholds: zero masks in the column-major copy
+        colm[j * kMaxThresholds + t] = T(0);
+        colb[j * kMaxThresholds + t] = T(0);
+        return;
+    }
     const float thr = th.t[t];
     const float hi = __fmul_rn(thr, (float)(uint64_t)num_vecs);
     const double lo = __dmul_rn(1.0 - (double)thr, (double)(uint64_t)num_vecs);
@@ -256,6 +261,30 @@ __device__ __forceinline__ uint32_t any_bit(T u)
     else return (uint32_t)u;
 }
 
+// lost + (d & m != 0 ? popc(m) : 0) as AND, (AND-OR), MIN, MAD_U24 with the
+// mask and its popcount in SGPRs (one scalar operand per instruction).
+__device__ __forceinline__ uint32_t lost_step(uint64_t d, uint64_t m, uint32_t pc, uint32_t lost)
+{
+    uint32_t z;
+    asm("v_and_b32 %0, %2, %3\n\t"
+        "v_and_or_b32 %0, %4, %5, %0\n\t"
+        "v_min_u32 %0, 1, %0\n\t"
+        "v_mad_u32_u24 %1, %0, %6, %1"
+        : "=&v"(z), "+v"(lost)
+        : "s"((uint32_t)m), "v"((uint32_t)d), "v"((uint32_t)(d >> 32)), "s"((uint32_t)(m >> 32)), "s"(pc));
+    return lost;
+}
+__device__ __forceinline__ uint32_t lost_step(uint32_t d, uint32_t m, uint32_t pc, uint32_t lost)
+{
+    uint32_t z;
+    asm("v_and_b32 %0, %2, %3\n\t"
+        "v_min_u32 %0, 1, %0\n\t"
+        "v_mad_u32_u24 %1, %0, %4, %1"
+        : "=&v"(z), "+v"(lost)
+        : "s"(m), "v"(d), "s"(pc));
+    return lost;
+}
+
 template <typename T, int NT, bool NESTED>
 __global__ __launch_bounds__(256) void score_rows_kernel(const T *__restrict__ data, int64_t num_vecs, int V,
                                                          const T *__restrict__ colm, const T *__restrict__ colb,
@@ -284,7 +313,6 @@ __global__ __launch_bounds__(256) void score_rows_kernel(const T *__restrict__ d
             if (t < nt) pcsum[t] += (uint32_t)popc((WideT<T>)colm[(int64_t)j * kMaxThresholds + t]);
 #pragma unroll
     for (int t = 0; t < NT; ++t) pcsum[t] = (uint32_t)wave_sum((int)pcsum[t]);
-    const uint4 zero4 = make_uint4(0, 0, 0, 0);
     for (int64_t r0 = wave * kWave; r0 < num_vecs; r0 += nwaves * kWave) {
         const int64_t r = r0 + lane;
         const bool ok = r < num_vecs;
@@ -292,12 +320,14 @@ __global__ __launch_bounds__(256) void score_rows_kernel(const T *__restrict__ d
         uint32_t lost[NT];
 #pragma unroll
         for (int t = 0; t < NT; ++t) lost[t] = 0;
+        // Chunk loads are unconditional (indices clamped to the row) so the
+        // next group's loads stay in flight while this group is scored.
         uint4 cur[G], nxt[G];
 #pragma unroll
-        for (int c = 0; c < G; ++c) cur[c] = c < chunks ? row[c] : zero4;
+        for (int c = 0; c < G; ++c) cur[c] = row[min(c, chunks - 1)];
         for (int c0 = 0; c0 < chunks; c0 += G) {
 #pragma unroll
-            for (int c = 0; c < G; ++c) nxt[c] = c0 + G + c < chunks ? row[c0 + G + c] : zero4;
+            for (int c = 0; c < G; ++c) nxt[c] = row[min(c0 + G + c, chunks - 1)];
 #pragma unroll
             for (int c = 0; c < G; ++c) {
                 if (c0 + c >= chunks) break; // uniform
@@ -305,22 +335,19 @@ __global__ __launch_bounds__(256) void score_rows_kernel(const T *__restrict__ d
                 for (int e = 0; e < EPC; ++e) {
                     const int64_t j = (int64_t)(c0 + c) * EPC + e;
                     const T x = chunk_elem<T>(cur[c], e);
+                    // the column's masks (zero past nt) and bitvals: scalar
+                    // loads issued together
                     const T *cm = colm + j * kMaxThresholds;
                     const T *cb = colb + j * kMaxThresholds;
-                    const T d0 = NESTED ? (T)(x ^ cb[0]) : T(0);
-                    T mprev = 0;
-                    uint32_t last = 0;
+                    T m[NT];
+#pragma unroll
+                    for (int t = 0; t < NT; ++t) m[t] = cm[t];
+                    const T d0 = (T)(x ^ cb[0]);
 #pragma unroll
                     for (int t = 0; t < NT; ++t) {
-                        if (t >= nt) break;
-                        const T m = cm[t];
-                        if (m == 0) continue; // no bits: nothing to lose
-                        if (m != mprev) {
-                            const T d = NESTED ? d0 : (T)(x ^ cb[t]);
-                            last = __umul24(min(any_bit<T>((T)(d & m)), 1u), (uint32_t)popc((WideT<T>)m));
-                            mprev = m;
-                        }
-                        lost[t] += last;
+                        if (m[t] == 0) continue; // uniform: no bits, nothing to lose
+                        const T d = NESTED ? d0 : (T)(x ^ cb[t]);
+                        lost[t] = lost_step(d, m[t], (uint32_t)popc((WideT<T>)m[t]), lost[t]);
                     }
                 }
             }
@@ -464,7 +491,7 @@ int preprocess_typed(const void *data, int64_t N, int64_t V, const Thresholds &t
     hipLaunchKernelGGL(reduce_counts_kernel, dim3((unsigned)rgrid), dim3(kWave * kReduceWaves), 0, stream,
                        (const int32_t *)part, row_blocks, V, 8 * (int)sizeof(T), cnt);
     IBP_LAUNCH_CHECK();
-    const int64_t mthreads = V * th.count;
+    const int64_t mthreads = V * kMaxThresholds;
     hipLaunchKernelGGL((create_masks_kernel<T>), dim3((unsigned)((mthreads + 255) / 256)), dim3(256), 0, stream, cnt, N,
                        V, th, masks, bitvals, colm, colb);
     IBP_LAUNCH_CHECK();
