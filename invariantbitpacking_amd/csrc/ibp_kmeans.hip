@@ -43,10 +43,13 @@ __global__ void km_gather_kernel(const T *data, int64_t V, const int64_t *rows, 
 }
 
 // Nearest centroid by popc(x ^ c); equal distances go to the highest index.
-// One wave per row; each element is loaded once and compared with up to 16
-// centroids (16 distance accumulators per lane), then the 16 distances are
-// summed over the wave.
+// One wave per group of kAssignRows rows; each centroid word is loaded once
+// per group (the centroids are re-read from L2 by every group, so more rows
+// per group means less L2 traffic) and compared with the group's elements,
+// 16 centroids at a time (16 x kAssignRows distance accumulators per lane);
+// each distance is then summed over the wave.
 constexpr int kAssignGroup = 16;
+constexpr int kAssignRows = 4;
 template <typename T>
 __global__ __launch_bounds__(256) void km_assign_kernel(const T *data, int64_t N, int64_t V, const T *centroids,
                                                         int K, int32_t *cluster)
@@ -54,30 +57,52 @@ __global__ __launch_bounds__(256) void km_assign_kernel(const T *data, int64_t N
     const int lane = lane_id();
     const int64_t wave = global_wave_id();
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
-    for (int64_t i = wave; i < N; i += nwaves) {
-        int best = 0;
-        int best_d = 0x7fffffff;
+    for (int64_t i0 = wave * kAssignRows; i0 < N; i0 += nwaves * kAssignRows) {
+        int best[kAssignRows], best_d[kAssignRows];
+#pragma unroll
+        for (int r = 0; r < kAssignRows; ++r) {
+            best[r] = 0;
+            best_d[r] = 0x7fffffff;
+        }
+        // rows past N repeat row i0 (valid memory); their result is dropped
+        int64_t row[kAssignRows];
+#pragma unroll
+        for (int r = 0; r < kAssignRows; ++r) row[r] = i0 + r < N ? i0 + r : i0;
         for (int c0 = 0; c0 < K; c0 += kAssignGroup) {
-            int d[kAssignGroup];
+            int d[kAssignGroup][kAssignRows];
 #pragma unroll
-            for (int g = 0; g < kAssignGroup; ++g) d[g] = 0;
+            for (int g = 0; g < kAssignGroup; ++g)
+#pragma unroll
+                for (int r = 0; r < kAssignRows; ++r) d[g][r] = 0;
             for (int64_t j = lane; j < V; j += kWave) {
-                const KW<T> x = (KW<T>)data[i * V + j];
+                KW<T> x[kAssignRows];
 #pragma unroll
-                for (int g = 0; g < kAssignGroup; ++g)
-                    if (c0 + g < K) d[g] += popc((KW<T>)(x ^ (KW<T>)centroids[(int64_t)(c0 + g) * V + j]));
+                for (int r = 0; r < kAssignRows; ++r) x[r] = (KW<T>)data[row[r] * V + j];
+#pragma unroll
+                for (int g = 0; g < kAssignGroup; ++g) {
+                    if (c0 + g >= K) break; // uniform
+                    const KW<T> cw = (KW<T>)centroids[(int64_t)(c0 + g) * V + j];
+#pragma unroll
+                    for (int r = 0; r < kAssignRows; ++r) d[g][r] += popc((KW<T>)(x[r] ^ cw));
+                }
             }
 #pragma unroll
             for (int g = 0; g < kAssignGroup; ++g) {
                 if (c0 + g >= K) break; // uniform
-                const int dist = wave_sum(d[g]);
-                if (dist <= best_d) {
-                    best_d = dist;
-                    best = c0 + g;
+#pragma unroll
+                for (int r = 0; r < kAssignRows; ++r) {
+                    const int dist = wave_sum(d[g][r]);
+                    if (dist <= best_d[r]) {
+                        best_d[r] = dist;
+                        best[r] = c0 + g;
+                    }
                 }
             }
         }
-        if (lane == 0) cluster[i] = best;
+        if (lane == 0)
+#pragma unroll
+            for (int r = 0; r < kAssignRows; ++r)
+                if (i0 + r < N) cluster[i0 + r] = best[r];
     }
 }
 
@@ -86,8 +111,15 @@ __global__ __launch_bounds__(256) void km_assign_kernel(const T *data, int64_t N
 // perm[offset[c] ..) = the rows of cluster c (in no particular order).  For
 // K <= kSortMaxK; larger K use the three-kernel form below.
 constexpr int kSortMaxK = 4096;
+//
+// It also cuts every cluster into chunks of at most `chunk_rows` rows for the
+// bit count: chunk e = (cluster, first row, end row) in chunks[3e..3e+2],
+// cluster c's chunks are chunk_base[c] .. chunk_base[c] + chunk_count[c] - 1,
+// and nchunks[0] is the total.
 __global__ __launch_bounds__(1024) void km_sort_kernel(const int32_t *cluster, int64_t N, int K, int32_t *count,
-                                                       int32_t *offset, int32_t *perm)
+                                                       int32_t *offset, int32_t *perm, int chunk_rows,
+                                                       int32_t *chunks, int32_t *chunk_base, int32_t *chunk_count,
+                                                       int32_t *nchunks)
 {
     __shared__ int32_t hist[kSortMaxK];
     __shared__ int32_t base[kSortMaxK];
@@ -96,11 +128,20 @@ __global__ __launch_bounds__(1024) void km_sort_kernel(const int32_t *cluster, i
     for (int64_t i = threadIdx.x; i < N; i += blockDim.x) atomicAdd(&hist[cluster[i]], 1);
     __syncthreads();
     if (threadIdx.x == 0) {
-        int32_t acc = 0;
+        int32_t acc = 0, e = 0;
         for (int c = 0; c < K; ++c) {
             base[c] = acc;
-            acc += hist[c];
+            const int32_t n = hist[c];
+            chunk_base[c] = e;
+            for (int32_t b = 0; b < n; b += chunk_rows, ++e) {
+                chunks[3 * e] = c;
+                chunks[3 * e + 1] = acc + b;
+                chunks[3 * e + 2] = acc + (b + chunk_rows < n ? b + chunk_rows : n);
+            }
+            chunk_count[c] = e - chunk_base[c];
+            acc += n;
         }
+        nchunks[0] = e;
     }
     __syncthreads();
     for (int c = threadIdx.x; c < K; c += blockDim.x) {
@@ -193,6 +234,70 @@ __global__ __launch_bounds__(256) void km_bitcount_kernel(const T *data, int64_t
         const int32_t v = sum[k * kWave + jl];
         if (gridDim.z == 1) *dst = v;
         else if (v) atomicAdd(dst, v);
+    }
+}
+
+// Bit counts of one chunk of a cluster's rows (km_sort_kernel's list):
+// grid = (column tiles, maximum chunk count); blocks past the list's length
+// return at once.  Same counting as km_bitcount_kernel; the block's counters
+// go to part[e][j*B + k], summed per cluster by km_reduce_bits_kernel.
+// Chunks of equal size keep every block's work alike however uneven the
+// clusters are.
+template <typename T>
+__global__ __launch_bounds__(256) void km_bitcount_chunks_kernel(const T *data, int64_t V, const int32_t *perm,
+                                                                 const int32_t *chunks, const int32_t *nchunks,
+                                                                 int32_t *part)
+{
+    constexpr int B = 8 * (int)sizeof(T);
+    constexpr int CHUNK = 15;
+    using W = KW<T>;
+    __shared__ int32_t sum[B * kWave];
+    const int e = blockIdx.y;
+    if (e >= nchunks[0]) return; // uniform for the block
+    const int lane = lane_id();
+    const int wv = wave_in_block_id();
+    const int64_t j = (int64_t)blockIdx.x * kWave + lane;
+    const bool col = j < V;
+    const int waves = blockDim.x / kWave;
+    const int64_t q0 = chunks[3 * e + 1], q1 = chunks[3 * e + 2];
+    for (int i = threadIdx.x; i < B * kWave; i += blockDim.x) sum[i] = 0;
+    int32_t acc[B];
+#pragma unroll
+    for (int k = 0; k < B; ++k) acc[k] = 0;
+    for (int64_t q = q0 + wv; q < q1; q += (int64_t)CHUNK * waves) {
+        W x[CHUNK];
+#pragma unroll
+        for (int u = 0; u < CHUNK; ++u) {
+            const int64_t qi = q + (int64_t)u * waves;
+            x[u] = (col && qi < q1) ? (W)data[(int64_t)perm[qi] * V + j] : W(0);
+        }
+        csa_count<W, B, CHUNK>(x, acc);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < B; ++k)
+        if (acc[k]) atomicAdd(&sum[k * kWave + lane], acc[k]);
+    __syncthreads();
+    int32_t *dst = part + (int64_t)e * V * B;
+    for (int i = threadIdx.x; i < B * kWave; i += blockDim.x) {
+        const int jl = i / B, k = i % B;
+        const int64_t jj = (int64_t)blockIdx.x * kWave + jl;
+        if (jj < V) dst[jj * B + k] = sum[k * kWave + jl];
+    }
+}
+
+// bits_set[(c*V + j)*B + k] = sum of part[e][j*B + k] over cluster c's chunks.
+__global__ void km_reduce_bits_kernel(int64_t VB, int K, const int32_t *chunk_base, const int32_t *chunk_count,
+                                      const int32_t *part, int32_t *bits_set)
+{
+    const int64_t total = (int64_t)K * VB;
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (int64_t)gridDim.x * blockDim.x) {
+        const int c = (int)(g / VB);
+        const int64_t i = g % VB;
+        const int32_t e0 = chunk_base[c], ne = chunk_count[c];
+        int32_t acc = 0;
+        for (int32_t e = 0; e < ne; ++e) acc += part[(int64_t)(e0 + e) * VB + i];
+        bits_set[g] = acc;
     }
 }
 
@@ -324,8 +429,19 @@ int kmeans_typed(const void *data_v, int64_t N, int64_t V, int K, int64_t seed, 
     }
     T *cent = nullptr, *maj = nullptr;
     int32_t *cluster = nullptr, *count = nullptr, *offset = nullptr, *fill = nullptr, *perm = nullptr, *bits = nullptr;
+    int32_t *chunks = nullptr, *part = nullptr;
     int64_t *rows = nullptr;
     const int64_t nbits = (int64_t)K * V * B;
+    // The LDS counting sort (one block) serves K <= kSortMaxK and up to 2^20
+    // rows; it also cuts the clusters into chunks of chunk_rows rows whose
+    // partial bit counts (at most 256 MiB of them) are summed per cluster.
+    const bool sort_in_lds = K <= kSortMaxK && N <= ((int64_t)1 << 20);
+    const int64_t per_chunk = V * B * (int64_t)sizeof(int32_t);
+    const int64_t max_chunks_mem = ((int64_t)256 << 20) / per_chunk;
+    const bool chunked = sort_in_lds && max_chunks_mem > K;
+    int64_t chunk_rows = 256;
+    if (chunked && N / chunk_rows + K + 1 > max_chunks_mem) chunk_rows = (N + max_chunks_mem - K - 1) / (max_chunks_mem - K);
+    const int64_t max_chunks = chunked ? N / chunk_rows + K + 1 : 0;
     auto cleanup = [&]() {
         (void)hipFreeAsync(cent, s);
         (void)hipFreeAsync(maj, s);
@@ -334,6 +450,8 @@ int kmeans_typed(const void *data_v, int64_t N, int64_t V, int K, int64_t seed, 
         (void)hipFreeAsync(perm, s);
         (void)hipFreeAsync(bits, s);
         (void)hipFreeAsync(rows, s);
+        (void)hipFreeAsync(chunks, s);
+        (void)hipFreeAsync(part, s);
     };
     hipError_t e = hipSuccess;
     if ((e = hipMallocAsync((void **)&cent, sizeof(T) * (size_t)(K * V), s)) != hipSuccess ||
@@ -342,7 +460,10 @@ int kmeans_typed(const void *data_v, int64_t N, int64_t V, int K, int64_t seed, 
         (e = hipMallocAsync((void **)&count, sizeof(int32_t) * 3 * (size_t)K, s)) != hipSuccess ||
         (e = hipMallocAsync((void **)&perm, sizeof(int32_t) * (size_t)(N > 0 ? N : 1), s)) != hipSuccess ||
         (e = hipMallocAsync((void **)&bits, sizeof(int32_t) * (size_t)nbits, s)) != hipSuccess ||
-        (e = hipMallocAsync((void **)&rows, sizeof(int64_t) * (size_t)K, s)) != hipSuccess) {
+        (e = hipMallocAsync((void **)&rows, sizeof(int64_t) * (size_t)K, s)) != hipSuccess ||
+        (chunked && (e = hipMallocAsync((void **)&chunks, sizeof(int32_t) * (size_t)(3 * max_chunks + 2 * K + 1), s)) !=
+                        hipSuccess) ||
+        (chunked && (e = hipMallocAsync((void **)&part, (size_t)(per_chunk * max_chunks), s)) != hipSuccess)) {
         cleanup();
         return set_error(IBP_ERR_NOMEM, "kmeans workspace allocation failed: %s", hipGetErrorString(e));
     }
@@ -353,33 +474,41 @@ int kmeans_typed(const void *data_v, int64_t N, int64_t V, int K, int64_t seed, 
     hipLaunchKernelGGL(km_zero_kernel, dim3(blocks_for(N)), dim3(256), 0, s, cluster, N);
     IBP_LAUNCH_CHECK();
 
-    int64_t assign_grid = (N * kWave + 255) / 256;
+    int64_t assign_grid = ((N + kAssignRows - 1) / kAssignRows * kWave + 255) / 256;
     if (assign_grid > (int64_t)dev.num_cus * 8) assign_grid = (int64_t)dev.num_cus * 8;
     if (assign_grid < 1) assign_grid = 1;
     const unsigned tiles = (unsigned)((V + kWave - 1) / kWave);
-    // Row splits of the bit count: about 64 rows of a cluster per wave (four
-    // waves per block); one split lets each block store its counters.
+    // Without the chunk list (very large N or K): row splits of the bit count,
+    // about 64 rows of a cluster per wave (four waves per block); one split
+    // lets each block store its counters.
     const int64_t per_cluster = N / (K > 0 ? K : 1);
     int64_t splits64 = (per_cluster + 255) / 256;
     if (splits64 < 1) splits64 = 1;
     if (splits64 > 64) splits64 = 64;
     const unsigned splits = (unsigned)splits64;
-    const bool sort_in_lds = K <= kSortMaxK && N <= ((int64_t)1 << 20);
+    int32_t *chunk_base = chunked ? chunks + 3 * max_chunks : nullptr;
+    int32_t *chunk_count = chunked ? chunk_base + K : nullptr;
+    int32_t *nchunks = chunked ? chunk_count + K : nullptr;
     for (int it = 0; it < 10; ++it) {
         hipLaunchKernelGGL((km_assign_kernel<T>), dim3((unsigned)assign_grid), dim3(256), 0, s, data, N, V, cent, K,
                            cluster);
-        if (sort_in_lds) {
-            hipLaunchKernelGGL(km_sort_kernel, dim3(1), dim3(1024), 0, s, cluster, N, K, count, offset, perm);
+        if (chunked) {
+            hipLaunchKernelGGL(km_sort_kernel, dim3(1), dim3(1024), 0, s, cluster, N, K, count, offset, perm,
+                               (int)chunk_rows, chunks, chunk_base, chunk_count, nchunks);
+            hipLaunchKernelGGL((km_bitcount_chunks_kernel<T>), dim3(tiles, (unsigned)max_chunks), dim3(256), 0, s,
+                               data, V, perm, (const int32_t *)chunks, (const int32_t *)nchunks, part);
+            hipLaunchKernelGGL(km_reduce_bits_kernel, dim3(blocks_for(nbits, 1 << 16)), dim3(256), 0, s, V * B, K,
+                               (const int32_t *)chunk_base, (const int32_t *)chunk_count, (const int32_t *)part, bits);
         } else {
             hipLaunchKernelGGL(km_zero_kernel, dim3(1), dim3(256), 0, s, count, (int64_t)K);
             hipLaunchKernelGGL(km_hist_kernel, dim3(blocks_for(N)), dim3(256), 0, s, cluster, N, count);
             hipLaunchKernelGGL(km_offsets_kernel, dim3(1), dim3(64), 0, s, count, K, offset, fill);
             hipLaunchKernelGGL(km_scatter_kernel, dim3(blocks_for(N)), dim3(256), 0, s, cluster, N, offset, fill,
                                perm);
+            if (splits > 1) hipLaunchKernelGGL(km_zero_kernel, dim3(blocks_for(nbits)), dim3(256), 0, s, bits, nbits);
+            hipLaunchKernelGGL((km_bitcount_kernel<T>), dim3(tiles, (unsigned)K, splits), dim3(256), 0, s, data, V,
+                               perm, offset, count, bits);
         }
-        if (splits > 1) hipLaunchKernelGGL(km_zero_kernel, dim3(blocks_for(nbits)), dim3(256), 0, s, bits, nbits);
-        hipLaunchKernelGGL((km_bitcount_kernel<T>), dim3(tiles, (unsigned)K, splits), dim3(256), 0, s, data, V, perm,
-                           offset, count, bits);
         hipLaunchKernelGGL((km_majority_kernel<T>), dim3(blocks_for((int64_t)K * V, 1 << 20)), dim3(256), 0, s, V, K,
                            count, bits, maj);
         hipLaunchKernelGGL((km_construct_kernel<T>), dim3((unsigned)((K * 32 + 255) / 256)), dim3(256), 0, s, V, K,
