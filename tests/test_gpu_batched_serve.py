@@ -102,6 +102,37 @@ def test_batched_graph_replays(ibp, sets):
     bf.close()
 
 
+@pytest.mark.parametrize("subset,itypes,segmented", [
+    ((0, 1), (torch.int64,), True),                 # 8-byte rows <= 64 elements, int64 indices
+    ((0, 1), (torch.int32,), True),                 # the same with int32 indices
+    ((3, 5), (torch.int64,), True),                 # 4-byte rows <= 64 elements
+    ((3, 5), (torch.int64, torch.int32), False),    # mixed index widths: persistent, one kind
+    ((2, 4), (torch.int64,), False),                # wide rows only: persistent, one kind
+])
+def test_batched_launch_forms(ibp, sets, subset, itypes, segmented):
+    """Each launch form of a plan (segmented per task, or persistent compiled
+    for the kinds present) against the plain fetch, eagerly, after autotuning
+    and over graph replays."""
+    import json
+    chosen = [sets[k] for k in subset]
+    tasks = _tasks(chosen, 7, itypes)
+    bf = ibp.BatchedFetch(tasks)
+    assert (json.loads(bf.describe())["segmented_blocks"] > 0) == segmented
+    bf.launch()
+    torch.cuda.synchronize()
+    _check(chosen, tasks)
+    bf.autotune()
+    bf.capture_graph()
+    for rep in range(3):
+        for t in tasks:
+            t["out"].fill_(rep - 5)
+        torch.cuda.synchronize()
+        bf.graph_launch()
+        torch.cuda.synchronize()
+        _check(chosen, tasks)
+    bf.close()
+
+
 def test_batched_out_of_range_rows_are_zero(ibp, sets):
     s = sets[0]
     n, v = s["x"].shape
