@@ -202,6 +202,13 @@ typedef struct ibp_fetch_task {
     int32_t index_bytes;
     int32_t comp_len; /* hint, <= 0 means vec_size */
     int32_t comp_loc; /* ibp_loc */
+    /* Condensed stream (as ibp_layout.offsets / stream_bytes): the inclusive
+     * end byte offset of each of the num_vecs stored rows in comp, or NULL for
+     * the slot layout; bitmask may then be NULL.  A plan with condensed tasks
+     * holds only condensed tasks, all with rows of <= 64 elements of one
+     * element width and one index width. */
+    const int64_t *offsets;
+    int64_t stream_bytes;
 } ibp_fetch_task;
 
 typedef struct ibp_plan ibp_plan;
@@ -210,7 +217,8 @@ typedef struct ibp_plan ibp_plan;
  * index/output buffers in place between launches).  The plan owns a device work
  * queue; all tasks run in ONE kernel launch: segmented (each task its own
  * blocks) when every task has rows of <= 64 elements of one element width and
- * one index width, else persistent over a work queue. */
+ * one index width (always for condensed tasks), else persistent over a work
+ * queue. */
 int ibp_plan_create(const ibp_fetch_task *tasks, int num_tasks, ibp_plan **plan);
 int ibp_plan_destroy(ibp_plan *plan);
 /* Launch on a stream (graph-capturable: no allocation, no memset node). */

@@ -20,7 +20,8 @@ class FetchTask(ctypes.Structure):
     """ibp_fetch_task."""
     _fields_ = [("comp", _P), ("mask", _P), ("bitval", _P), ("bitmask", _P), ("index", _P), ("out", _P),
                 ("num_vecs", _I64), ("vec_size", _I64), ("n", _I64), ("elem_bytes", ctypes.c_int32),
-                ("index_bytes", ctypes.c_int32), ("comp_len", ctypes.c_int32), ("comp_loc", ctypes.c_int32)]
+                ("index_bytes", ctypes.c_int32), ("comp_len", ctypes.c_int32), ("comp_loc", ctypes.c_int32),
+                ("offsets", _P), ("stream_bytes", _I64)]
 
 
 class Layout(ctypes.Structure):

@@ -1,4 +1,4 @@
-"""Batched fetch: many tables fetched and unpacked by ONE persistent kernel.
+"""Batched fetch: many tables fetched and unpacked by ONE kernel launch.
 
 No reference counterpart (the reference fetches one table per call); each
 table's result is exactly what decompress_fetch returns for it.  The plan
@@ -14,7 +14,12 @@ from . import _capi
 
 class BatchedFetch:
     """tasks: list of dicts with keys comp, mask, bitval, bitmask, out and
-    optionally index (int32/int64 GPU tensor), comp_len (int)."""
+    optionally index (int32/int64 GPU tensor), comp_len (int).
+
+    Condensed streams (compress_condensed): comp is the 1-D stream, with
+    offsets (its int64 byte offsets, GPU or pinned) and vec_size in place of
+    bitmask; a plan's tasks are then all condensed, with rows of <= 64
+    elements of one element width and one index width."""
 
     def __init__(self, tasks):
         self._keep = []
@@ -22,7 +27,17 @@ class BatchedFetch:
         for i, t in enumerate(tasks):
             comp, out = t["comp"], t["out"]
             index = t.get("index")
-            num_vecs, v = (1, comp.size(0)) if comp.dim() == 1 else tuple(comp.shape)
+            offsets = t.get("offsets")
+            if offsets is not None:
+                if offsets.dtype != torch.int64 or offsets.dim() != 1 or not offsets.is_contiguous():
+                    raise RuntimeError("task %d: offsets must be a contiguous 1-D int64 tensor" % i)
+                if not (offsets.is_cuda or offsets.is_pinned()):
+                    raise RuntimeError("task %d: offsets must be on the GPU or pinned" % i)
+                if comp.dim() != 1 or not comp.is_contiguous():
+                    raise RuntimeError("task %d: a condensed stream must be a contiguous 1-D tensor" % i)
+                num_vecs, v = offsets.numel(), int(t["vec_size"])
+            else:
+                num_vecs, v = (1, comp.size(0)) if comp.dim() == 1 else tuple(comp.shape)
             n = num_vecs if index is None else index.numel()
             if out.shape != (n, v) or out.dtype != comp.dtype or not out.is_contiguous():
                 raise RuntimeError("task %d: output must be a contiguous [%d, %d] %s tensor" % (i, n, v, comp.dtype))
@@ -32,12 +47,15 @@ class BatchedFetch:
                 if index.dtype not in (torch.int32, torch.int64):
                     index = index.to(torch.int64)
                 index = index.contiguous()
-            self._keep += [comp, t["mask"], t["bitval"], t["bitmask"], out, index]
+            bitmask = t.get("bitmask")
+            self._keep += [comp, t["mask"], t["bitval"], bitmask, out, index, offsets]
             arr[i] = _capi.FetchTask(
-                comp.data_ptr(), t["mask"].data_ptr(), t["bitval"].data_ptr(), t["bitmask"].data_ptr(),
+                comp.data_ptr(), t["mask"].data_ptr(), t["bitval"].data_ptr(),
+                None if bitmask is None else bitmask.data_ptr(),
                 None if index is None else index.data_ptr(), out.data_ptr(), num_vecs, v, n,
                 comp.element_size(), 0 if index is None else index.element_size(), int(t.get("comp_len") or 0),
-                1 if comp.is_cuda else 2)
+                1 if comp.is_cuda else 2, None if offsets is None else offsets.data_ptr(),
+                0 if offsets is None else comp.numel() * comp.element_size())
         self._tasks = arr
         self._plan = ctypes.c_void_p()
         _capi.call("ibp_plan_create", arr, len(tasks), ctypes.byref(self._plan))

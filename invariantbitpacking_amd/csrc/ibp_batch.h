@@ -44,6 +44,7 @@ struct BatchShape {
     int seg_blocks = 0;  // 0: persistent launch over work items
     int elem_bytes = 8;  // segmented launch
     int index_bytes = 8; // segmented launch (8 also for tasks without index)
+    int layout = kLayoutSlots; // kLayoutCondensed: condensed tasks (always segmented)
 };
 
 // A launch shape for `total_rows` rows: work items small enough that every

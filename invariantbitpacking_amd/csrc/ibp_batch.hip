@@ -150,6 +150,33 @@ __global__ __launch_bounds__(256) void batch_segmented_kernel(const TaskDev *__r
     fetch_small_range<T, IdxT, ROWS>(p, col, wave * ROWS, p.n, nwaves * ROWS);
 }
 
+// Segmented launch of condensed tasks: as batch_segmented_kernel, with the
+// condensed fetch's loop (two offsets per row, reads bounded by the row's
+// range and the stream).
+template <typename T, typename IdxT>
+__global__ __launch_bounds__(256) void batch_segmented_condensed_kernel(const TaskDev *__restrict__ tasks, int ntasks)
+{
+    constexpr int ROWS = 8;
+    const int blk = (int)blockIdx.x;
+    int lo = 0, hi = ntasks - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (tasks[mid].block_begin <= blk) lo = mid;
+        else hi = mid - 1;
+    }
+    const TaskDev &t = tasks[lo];
+    const FetchParams &p = t.p;
+    const SmallCols<T> col = load_small_cols<T>(p.mask, p.bitval, (int)p.vec_size);
+    const int64_t waves_per_block = blockDim.x / kWave;
+    const int64_t wave = (int64_t)(blk - t.block_begin) * waves_per_block + wave_in_block_id();
+    const int64_t nwaves = (int64_t)t.blocks * waves_per_block;
+    const bool nt = rows_fill_lines(p.vec_size, (int)sizeof(T));
+    with_runs<T>(col, nt, [&](auto runs, auto ntc) {
+        fetch_small_range_meta<T, IdxT, ROWS, kLayoutCondensed, decltype(ntc)::value, decltype(runs)::value>(
+            p, col, wave * ROWS, p.n, nwaves * ROWS);
+    });
+}
+
 BatchConfig adapt_config(const BatchConfig &base, int64_t total_rows, int num_cus)
 {
     BatchConfig c = base;
@@ -186,7 +213,9 @@ int64_t fill_tasks(const ibp_fetch_task *tasks, int n, const BatchConfig &cfg, i
         d.p.n = t.n;
         const bool host = t.comp_loc == IBP_LOC_HOST || (t.comp_loc == IBP_LOC_AUTO && pointer_is_host(t.comp));
         d.p.first_words = first_words_for(t.vec_size, t.elem_bytes, t.comp_len, host);
-        d.p.err = t.index ? index_error_counter() : nullptr;
+        d.p.err = (t.index || t.offsets) ? index_error_counter() : nullptr;
+        d.p.offsets = t.offsets;
+        d.p.stream_bytes = t.stream_bytes;
         d.elem_bytes = t.elem_bytes;
         d.index_bytes = t.index_bytes;
         d.small = t.vec_size <= kWave;
@@ -200,8 +229,12 @@ int64_t fill_tasks(const ibp_fetch_task *tasks, int n, const BatchConfig &cfg, i
     }
     BatchShape sh;
     sh.kinds = kinds ? kinds : kTasksAll;
-    if (cfg.segmented && (kinds == kTasksSmall8 || kinds == kTasksSmall4) && (idx_mask == 1 || idx_mask == 2) &&
-        rows > 0) {
+    // Condensed tasks (checked at plan creation: all small, one width, one
+    // index width) have only the segmented form.
+    const bool condensed = n > 0 && tasks[0].offsets != nullptr;
+    if (condensed) sh.layout = kLayoutCondensed;
+    if ((cfg.segmented || condensed) && (kinds == kTasksSmall8 || kinds == kTasksSmall4) &&
+        (idx_mask == 1 || idx_mask == 2) && rows > 0) {
         // Blocks per task: one group of rows per wave, as the plain fetch
         // launches, shrunk in proportion to the task's rows when the total
         // exceeds max_blocks (each task with rows keeps at least one block).
@@ -229,8 +262,13 @@ int64_t fill_tasks(const ibp_fetch_task *tasks, int n, const BatchConfig &cfg, i
 }
 
 template <typename T, typename IdxT>
-void launch_segmented(int rows, int blocks, hipStream_t s, const TaskDev *dev_tasks, int ntasks)
+void launch_segmented(int rows, int blocks, hipStream_t s, const TaskDev *dev_tasks, int ntasks, int layout)
 {
+    if (layout == kLayoutCondensed) {
+        hipLaunchKernelGGL((batch_segmented_condensed_kernel<T, IdxT>), dim3(blocks), dim3(256), 0, s, dev_tasks,
+                           ntasks);
+        return;
+    }
     if (rows == 8)
         hipLaunchKernelGGL((batch_segmented_kernel<T, IdxT, 8>), dim3(blocks), dim3(256), 0, s, dev_tasks, ntasks);
     else
@@ -275,11 +313,11 @@ int launch_batch(const TaskDev *dev_tasks, int ntasks, int64_t total_items, unsi
     if (shape.seg_blocks > 0) {
         const int rows = seg_rows(cfg), b = shape.seg_blocks;
         if (shape.elem_bytes == 8) {
-            if (shape.index_bytes == 4) launch_segmented<uint64_t, int32_t>(rows, b, s, dev_tasks, ntasks);
-            else launch_segmented<uint64_t, int64_t>(rows, b, s, dev_tasks, ntasks);
+            if (shape.index_bytes == 4) launch_segmented<uint64_t, int32_t>(rows, b, s, dev_tasks, ntasks, shape.layout);
+            else launch_segmented<uint64_t, int64_t>(rows, b, s, dev_tasks, ntasks, shape.layout);
         } else {
-            if (shape.index_bytes == 4) launch_segmented<uint32_t, int32_t>(rows, b, s, dev_tasks, ntasks);
-            else launch_segmented<uint32_t, int64_t>(rows, b, s, dev_tasks, ntasks);
+            if (shape.index_bytes == 4) launch_segmented<uint32_t, int32_t>(rows, b, s, dev_tasks, ntasks, shape.layout);
+            else launch_segmented<uint32_t, int64_t>(rows, b, s, dev_tasks, ntasks, shape.layout);
         }
         IBP_LAUNCH_CHECK();
         return IBP_OK;
@@ -358,7 +396,18 @@ extern "C" int ibp_plan_create(const ibp_fetch_task *tasks, int num_tasks, ibp_p
         IBP_CHECK(t.vec_size > 0 && t.n >= 0 && t.num_vecs >= 0, "task %d: invalid shape", i);
         IBP_CHECK(t.index != nullptr || t.n <= t.num_vecs, "task %d: n exceeds num_vecs without an index", i);
         IBP_CHECK(t.index == nullptr || t.index_bytes == 4 || t.index_bytes == 8, "task %d: index must be int32/int64", i);
-        IBP_CHECK(t.n == 0 || (t.comp && t.mask && t.bitval && t.bitmask && t.out), "task %d: null pointer", i);
+        IBP_CHECK(t.n == 0 || (t.comp && t.mask && t.bitval && (t.bitmask || t.offsets) && t.out),
+                  "task %d: null pointer", i);
+        if (tasks[0].offsets != nullptr || t.offsets != nullptr) {
+            const ibp_fetch_task &t0 = tasks[0];
+            IBP_CHECK(t.offsets != nullptr && t0.offsets != nullptr,
+                      "task %d: a plan's tasks are all condensed or all in the slot layout", i);
+            IBP_CHECK(t.vec_size <= 64 && t.elem_bytes == t0.elem_bytes &&
+                          (t.index ? t.index_bytes : 8) == (t0.index ? t0.index_bytes : 8),
+                      "task %d: condensed tasks need rows of <= 64 elements of one element width and one index width",
+                      i);
+            IBP_CHECK(t.stream_bytes >= 0, "task %d: stream_bytes must be >= 0", i);
+        }
     }
     ibp_plan *pl = new ibp_plan();
     pl->tasks.assign(tasks, tasks + num_tasks);

@@ -207,6 +207,8 @@ struct ibp_server {
             t.index_bytes = 8;
             t.comp_len = s.comp_len;
             t.comp_loc = s.comp_loc;
+            t.offsets = nullptr;
+            t.stream_bytes = 0;
             tasks.push_back(t);
             nrows += r.n;
             r.st = St::Running;

@@ -167,3 +167,32 @@ def test_read_shared_loads_file_into_shared_memory(tmp_path):
         ibp.read_shared(str(f), [999, 37], torch.int64)
     with pytest.raises(RuntimeError, match="cannot open"):
         ibp.read_shared(str(tmp_path / "missing.bin"), [4], torch.int64)
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """Every struct the C ABI passes by pointer has the same size and field
+    offsets in include/ibp.h (compiled here with gcc) and in the ctypes
+    mirror the Python package uses."""
+    import shutil
+    import subprocess
+    from invariantbitpacking_amd import _capi
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    structs = {"ibp_fetch_task": _capi.FetchTask, "ibp_layout": _capi.Layout,
+               "ibp_server_config": _capi.ServerConfig, "ibp_server_stats": _capi.ServerStats}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "ibp.h"', "int main(void) {"]
+    for name, cls in structs.items():
+        lines.append('printf("%s sizeof %%zu\\n", sizeof(%s));' % (name, name))
+        for field, _ in cls._fields_:
+            lines.append('printf("%s %s %%zu\\n", offsetof(%s, %s));' % (name, field, name, field))
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = dict(l.rsplit(" ", 1) for l in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                        text=True).stdout.splitlines())
+    for name, cls in structs.items():
+        assert int(got["%s sizeof" % name]) == ctypes.sizeof(cls), name
+        for field, _ in cls._fields_:
+            assert int(got["%s %s" % (name, field)]) == getattr(cls, field).offset, (name, field)

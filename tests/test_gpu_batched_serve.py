@@ -1,4 +1,4 @@
-"""GPU parity of the batched (one persistent launch, HIP graph) and served
+"""GPU parity of the batched (one launch per plan, HIP graph) and served
 (continuous batching) forms: every result equals the plain decompress_fetch
 of the same rows, which the parity suite pins to the oracle."""
 import numpy as np
@@ -210,3 +210,61 @@ def test_server_admission_waits_for_free_hbm(ibp, sets):
     with pytest.raises(RuntimeError, match="can never fit"):
         srv.submit(wid, np.zeros(1 << 16, dtype=np.int64))
     srv.close()
+
+
+@pytest.mark.parametrize("w,shapes,itype", [
+    (8, [(3000, 64, "device"), (2000, 50, "pinned"), (1000, 16, "device")], torch.int64),
+    (4, [(1500, 48, "device"), (800, 64, "pinned")], torch.int32),
+])
+def test_batched_condensed_streams(ibp, w, shapes, itype):
+    """Condensed streams of several tables in one plan (segmented launch)
+    equal the original rows: eager, after autotuning and over graph replays,
+    streams in HBM or pinned memory."""
+    g = torch.Generator().manual_seed(11)
+    chosen, tasks = [], []
+    for k, (n, v, where) in enumerate(shapes):
+        s = make_set(ibp, n, v, w, seed=40 + k)
+        stream, offs = ibp.compress_condensed(s["x"].cuda(), s["mask"], s["bitval"])
+        torch.cuda.synchronize()
+        if where == "pinned":
+            stream, offs = stream.cpu().pin_memory(), offs.cpu().pin_memory()
+        rows = int(torch.randint(1, 2 * n, (1,), generator=g))
+        idx = torch.randint(0, n, (rows,), generator=g).to(itype).cuda()
+        out = torch.full((rows, v), -3, dtype=s["x"].dtype, device="cuda")
+        chosen.append(s)
+        tasks.append(dict(comp=stream, offsets=offs, vec_size=v, mask=s["mask"], bitval=s["bitval"], index=idx,
+                          out=out, comp_len=s["comp_len"]))
+    bf = ibp.BatchedFetch(tasks)
+    assert '"segmented_blocks": 0' not in bf.describe()
+    bf.launch()
+    torch.cuda.synchronize()
+    _check(chosen, tasks)
+    bf.autotune()
+    bf.capture_graph()
+    for rep in range(3):
+        for s, t in zip(chosen, tasks):
+            t["index"].copy_(torch.randint(0, s["x"].shape[0], t["index"].shape, generator=g).to(itype))
+            t["out"].fill_(rep)
+        torch.cuda.synchronize()
+        bf.graph_launch()
+        torch.cuda.synchronize()
+        _check(chosen, tasks)
+    bf.close()
+
+
+def test_batched_condensed_plan_rules(ibp, sets):
+    """A plan holds only condensed tasks of rows <= 64 elements of one width."""
+    s = sets[0]
+    stream, offs = ibp.compress_condensed(s["x"].cuda(), s["mask"], s["bitval"])
+    n, v = s["x"].shape
+    out = torch.empty((n, v), dtype=s["x"].dtype, device="cuda")
+    cond = dict(comp=stream, offsets=offs, vec_size=v, mask=s["mask"], bitval=s["bitval"], out=out)
+    slot = dict(comp=s["comp"], mask=s["mask"], bitval=s["bitval"], bitmask=s["bitmask"], out=out.clone())
+    with pytest.raises(RuntimeError, match="all condensed"):
+        ibp.BatchedFetch([cond, slot])
+    wide = sets[2]
+    wstream, woffs = ibp.compress_condensed(wide["x"].cuda(), wide["mask"], wide["bitval"])
+    wout = torch.empty(tuple(wide["x"].shape), dtype=wide["x"].dtype, device="cuda")
+    with pytest.raises(RuntimeError, match="<= 64 elements"):
+        ibp.BatchedFetch([dict(comp=wstream, offsets=woffs, vec_size=wide["x"].shape[1], mask=wide["mask"],
+                               bitval=wide["bitval"], out=wout)])
