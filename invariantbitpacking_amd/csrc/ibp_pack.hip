@@ -178,14 +178,19 @@ struct PackParams {
     uint32_t *bitmask; // may be nullptr with a condensed output
     void *gbuf;        // per-wave slot buffers when they do not fit LDS
     unsigned long long *err;
+    int split = 1;     // waves of a block sharing one 32-row group (1: a wave per group)
 };
 
 // Pack kernel.  BUF_LDS: the per-wave slot buffer is in LDS (dynamic shared
 // memory); otherwise in p.gbuf.  CONDENSED: output row i goes to bytes
 // [offsets[i-1], offsets[i]) of the stream, cut at its size; otherwise to the
 // V-element slot i.
+//
+// Groups of 32 rows (one bitmask word each) go to one wave, or, with
+// p.split > 1 (few groups, wide rows), to the p.split waves of a block, which
+// take every p.split-th row and combine the group's bitmask word in LDS.
 template <typename T, typename IdxT, bool BUF_LDS, bool CONDENSED>
-__global__ __launch_bounds__(256) void pack_kernel(PackParams p)
+__global__ __launch_bounds__(1024) void pack_kernel(PackParams p)
 {
     constexpr int B = Bits<T>::B;
     constexpr int W = (int)sizeof(T);
@@ -212,10 +217,16 @@ __global__ __launch_bounds__(256) void pack_kernel(PackParams p)
     if (small) col = load_small_cols<T>(p.mask, p.bitval, (int)V);
     const int len_raw = col.valid ? B : 0;
 
-    for (int64_t g = wave; g < ngroups; g += nwaves) {
+    const int split = p.split;
+    __shared__ uint32_t group_bits;
+    for (int64_t g = split > 1 ? (int64_t)blockIdx.x : wave; g < ngroups; g += split > 1 ? (int64_t)gridDim.x : nwaves) {
+        if (split > 1) {
+            if (threadIdx.x == 0) group_bits = 0;
+            __syncthreads();
+        }
         uint32_t bits = 0;
         const int64_t iend = (g + 1) * 32 < p.n ? (g + 1) * 32 : p.n;
-        for (int64_t i = g * 32; i < iend; ++i) {
+        for (int64_t i = g * 32 + (split > 1 ? wave_in_block : 0); i < iend; i += split) {
             const int64_t r = readfirstlane(row_of<IdxT>(p.index, i));
             const int64_t c = (r >= 0 && r < p.num_vecs) ? mask_row_of(p.clusters, p.num_clusters, r) : -1;
             if (c < 0) {
@@ -313,7 +324,14 @@ __global__ __launch_bounds__(256) void pack_kernel(PackParams p)
                 dst[k] = __hip_atomic_load(&buf[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             bits |= 1u << (i - g * 32);
         }
-        if (lane == 0 && p.bitmask) p.bitmask[g] = bits;
+        if (split > 1) {
+            if (lane == 0 && bits) atomicOr(&group_bits, bits);
+            __syncthreads();
+            if (threadIdx.x == 0 && p.bitmask) p.bitmask[g] = group_bits;
+            __syncthreads();
+        } else if (lane == 0 && p.bitmask) {
+            p.bitmask[g] = bits;
+        }
     }
     if (p.err && bad && lane == 0) atomicAdd(p.err, bad);
 }
@@ -394,8 +412,9 @@ int size_typed(const void *data, int64_t num_vecs, int64_t V, const void *mask, 
                uint64_t *total, hipStream_t stream)
 {
     const DeviceInfo &dev = device_info();
+    // 16 blocks of four waves per CU (8: 13% slower on 32768 x 1024 words)
     int64_t grid = (n * kWave + 255) / 256;
-    if (grid > (int64_t)dev.num_cus * 8) grid = (int64_t)dev.num_cus * 8;
+    if (grid > (int64_t)dev.num_cus * 16) grid = (int64_t)dev.num_cus * 16;
     if (grid < 1) grid = 1;
     hipLaunchKernelGGL((size_kernel<T, IdxT>), dim3((unsigned)grid), dim3(256), 0, stream, (const T *)data,
                        num_vecs, V, (const T *)mask, (const T *)bitval, clusters, num_clusters, index, n, sizes,
@@ -452,6 +471,30 @@ int pack_typed(PackParams p, hipStream_t stream)
     const int64_t buf_bytes = slot_buf_bytes(p.V, (int)sizeof(T));
     const int64_t lds_budget = 64 * 1024;
     const int64_t ngroups = (p.n + 31) / 32;
+    if (ngroups < (int64_t)dev.num_cus * 8) {
+        // Too few 32-row groups for a wave each: the waves of a block share a
+        // group (16 per block with slot buffers in device memory, as many as
+        // the LDS budget holds otherwise).
+        const bool lds = buf_bytes * 8 <= lds_budget;
+        int wpb = lds ? (int)(lds_budget / buf_bytes) : 16;
+        if (wpb > 16) wpb = 16;
+        p.split = wpb;
+        const int64_t grid = ngroups > 0 ? ngroups : 1;
+        if (lds) {
+            p.gbuf = nullptr;
+            hipLaunchKernelGGL((pack_kernel<T, IdxT, true, CONDENSED>), dim3((unsigned)grid), dim3(wpb * kWave),
+                               (size_t)(wpb * buf_bytes), stream, p);
+            IBP_LAUNCH_CHECK();
+            return IBP_OK;
+        }
+        IBP_HIP(hipMallocAsync(&p.gbuf, (size_t)(grid * wpb * buf_bytes), stream));
+        hipLaunchKernelGGL((pack_kernel<T, IdxT, false, CONDENSED>), dim3((unsigned)grid), dim3(wpb * kWave), 0,
+                           stream, p);
+        hipError_t e = hipGetLastError();
+        IBP_HIP(hipFreeAsync(p.gbuf, stream));
+        if (e != hipSuccess) return set_error(IBP_ERR_HIP, "pack kernel launch failed: %s", hipGetErrorString(e));
+        return IBP_OK;
+    }
     if (buf_bytes <= lds_budget) {
         int wpb = (int)(lds_budget / buf_bytes);
         if (wpb > 4) wpb = 4;
