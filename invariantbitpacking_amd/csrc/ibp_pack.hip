@@ -239,10 +239,14 @@ __global__ __launch_bounds__(1024) void pack_kernel(PackParams p)
             T xs = T(0);
             bool fs = false;
             int saved;
+            int incl = 0; // small rows: inclusive scan of the payload lengths
             if (small) {
                 xs = col.valid ? src[lane] : T(0);
                 fs = col.valid && (T)(xs & T(~col.free)) == col.bitval;
-                saved = wave_sum(fs ? B - col.len_flagged : 0);
+                // The payload scan gives the size as well: the bits saved are
+                // V*B minus the payload length.
+                incl = wave_incl_scan(fs ? col.len_flagged : len_raw);
+                saved = (int)V * B - readlane(incl, kWave - 1);
             } else {
                 saved = row_saved_bits<T>(src, mask, bitval, V);
             }
@@ -282,7 +286,7 @@ __global__ __launch_bounds__(1024) void pack_kernel(PackParams p)
                     if (hw) or_word<T>(buf, lane, hw);
                 }
                 const int L = fs ? col.len_flagged : len_raw;
-                const unsigned pbit = (unsigned)(wave_incl_scan(L) - L);
+                const unsigned pbit = (unsigned)(incl - L);
                 const T v = fs ? small_extract<T>(col, xs) : xs;
                 const int64_t q = Hw + pbit / B;
                 const int o = (int)(pbit % B);
