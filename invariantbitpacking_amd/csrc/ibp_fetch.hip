@@ -29,7 +29,16 @@ int fetch_launch(const FetchParams &p, int elem_bytes, int index_bytes, int impl
     const DeviceInfo &dev = device_info();
     // Blocks are at most 256 threads (four waves); larger requests are capped.
     const int block = (nthreads > 0 && nthreads <= 256) ? nthreads : 256;
-    const int rows = g_rows_in_flight.load();
+    int rows = g_rows_in_flight.load();
+#ifndef IBP_NO_SMALL_CALL_ROWS
+    // Small calls (few rows): fewer rows per wave, so that the grid still
+    // holds about two waves per SIMD; such a call is bound by the latency of
+    // its index -> slot -> store chain, not by bandwidth.
+    if (p.vec_size <= kWave && elem_bytes >= 4) {
+        const int64_t want = (int64_t)dev.num_cus * 8;
+        while (rows > 2 && (p.n + rows - 1) / rows < want) rows /= 2;
+    }
+#endif
     int64_t waves_needed = p.vec_size <= kWave ? (p.n + rows - 1) / rows : p.n;
     int64_t grid = nblks > 0 ? nblks : (waves_needed * kWave + block - 1) / block;
     int64_t cap = (int64_t)dev.num_cus * g_blocks_per_cu.load();
