@@ -453,7 +453,7 @@ extern "C" int ibp_plan_autotune(ibp_plan *pl, void *stream, float *best_ms)
     const DeviceInfo &dev = device_info();
     std::vector<BatchConfig> cands;
     for (int rif : {4, 8})
-        for (int mult : {4, 8, 16}) cands.push_back(BatchConfig{128, 2, rif, dev.num_cus * mult, 1, 0, 1});
+        for (int mult : {4, 8, 16, 32}) cands.push_back(BatchConfig{128, 2, rif, dev.num_cus * mult, 1, 0, 1});
     for (int rpi : {32, 64, 128, 256})
         for (int rif : {4, 8})
             for (int mult : {8, 16}) {
