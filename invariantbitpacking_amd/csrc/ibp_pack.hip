@@ -219,6 +219,14 @@ __global__ __launch_bounds__(1024) void pack_kernel(PackParams p)
 
     const int split = p.split;
     __shared__ uint32_t group_bits;
+    // LDS slot buffers of 4- and 8-byte words are zeroed once: the read-back
+    // of each packed row exchanges every word with zero, leaving the buffer
+    // clean for the next row.  Otherwise every packed row zeroes it first.
+    constexpr bool XCHG = BUF_LDS && sizeof(T) >= 4;
+    if constexpr (XCHG) {
+        for (int64_t k = lane; k < buf_bytes / 4; k += kWave) ((uint32_t *)bufb)[k] = 0u;
+        slot_fence<BUF_LDS>();
+    }
     for (int64_t g = split > 1 ? (int64_t)blockIdx.x : wave; g < ngroups; g += split > 1 ? (int64_t)gridDim.x : nwaves) {
         if (split > 1) {
             if (threadIdx.x == 0) group_bits = 0;
@@ -276,8 +284,10 @@ __global__ __launch_bounds__(1024) void pack_kernel(PackParams p)
                 }
                 continue;
             }
-            for (int64_t k = lane; k < buf_bytes / 4; k += kWave) ((uint32_t *)bufb)[k] = 0u;
-            slot_fence<BUF_LDS>();
+            if constexpr (!XCHG) {
+                for (int64_t k = lane; k < buf_bytes / 4; k += kWave) ((uint32_t *)bufb)[k] = 0u;
+                slot_fence<BUF_LDS>();
+            }
             int64_t cursor = 0;
             if (small) {
                 const uint64_t hb = ballot(fs);
@@ -324,8 +334,13 @@ __global__ __launch_bounds__(1024) void pack_kernel(PackParams p)
             }
             slot_fence<BUF_LDS>();
             const int64_t words = size / W; // the rest of the slot is zero
-            for (int64_t k = lane; k < (CONDENSED ? words : V); k += kWave)
-                dst[k] = __hip_atomic_load(&buf[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            for (int64_t k = lane; k < (CONDENSED ? words : V); k += kWave) {
+                if constexpr (XCHG)
+                    dst[k] = __hip_atomic_exchange(&buf[k], T(0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                else
+                    dst[k] = __hip_atomic_load(&buf[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            if constexpr (XCHG) slot_fence<BUF_LDS>(); // the exchanges land before the next row's ORs
             bits |= 1u << (i - g * 32);
         }
         if (split > 1) {
