@@ -1,4 +1,4 @@
-// ibp_batch.h -- internal interface of the persistent batched fetch kernel,
+// ibp_batch.h -- internal interface of the batched fetch kernels,
 // shared by the batch plan (ibp_batch.hip) and the server (ibp_serve.hip).
 #pragma once
 #include <hip/hip_runtime.h>

@@ -1,14 +1,18 @@
 // ibp_batch.hip -- batched fetch: many tables of uneven row counts and widths
-// fetched and unpacked by ONE persistent kernel from a device-side work queue.
+// fetched and unpacked by ONE kernel launch.
 //
 // No reference counterpart (the reference issues one decompress_fetch per
 // table, e.g. tests/dlrm_comp.py:79-81); values are those of
 // ibp_decompress_fetch on each table.
 //
-// Work item = (task, range of rows).  Waves claim items with one atomic on the
-// queue head; the head is reset by a small kernel launched before the
-// persistent kernel on the same stream, so a launch is two kernel nodes and can
-// be captured into a HIP graph (no memset node).
+// Two launch forms.  Segmented: when every task has rows of <= 64 elements of
+// one element width and one index width (always for condensed streams), each
+// task owns a range of blocks that runs the plain fetch's loop over its rows.
+// Persistent: otherwise, waves take work items = (task, range of rows), by
+// static striding or by claiming them with one atomic on a queue head; the
+// head is reset by a small kernel launched before on the same stream, so a
+// launch is at most two kernel nodes and can be captured into a HIP graph (no
+// memset node).
 #include <stdio.h>
 #include <string.h>
 

@@ -1,6 +1,6 @@
 // ibp_fetch_dev.h -- device routines that fetch and unpack rows of the IBP
-// slot format.  Shared by the plain fetch kernels (ibp_fetch.hip) and the
-// persistent batched kernel (ibp_batch.hip).
+// slot format.  Shared by the plain fetch kernels (ibp_fetch_launch.h) and the
+// batched kernels (ibp_batch.hip).
 //
 // Slot format (reference: ibp_compress_dev.cuh:13-15,61-92; decode
 // ibp_decompress_dev.cuh:279-336): V words of T.  A packed slot starts with

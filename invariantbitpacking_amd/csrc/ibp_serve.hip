@@ -4,7 +4,7 @@
 // issue decompress_fetch per request).  A request = (registered tensor set,
 // host index array).  Requests are admitted when their output fits the
 // device arena (admission by free HBM), merged with the other admitted
-// requests into ONE persistent-kernel launch, and launched on one of
+// requests into ONE batched launch (ibp_batch.hip), and launched on one of
 // `num_streams` streams as soon as a stream is free, so requests that arrive
 // while others run are batched and overlap on the device.  Each request's
 // output region and index range are its own, so a result never depends on

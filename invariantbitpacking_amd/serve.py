@@ -2,7 +2,7 @@
 
 No reference counterpart.  Requests (a registered compressed tensor set plus
 host row indices) are admitted when their output fits the server's device
-arena, merged with whatever else is admitted into one persistent-kernel launch
+arena, merged with whatever else is admitted into one batched kernel launch
 and run on one of several streams, so requests that arrive while others are on
 the GPU overlap with them.  Results are bit-identical to decompress_fetch.
 """
