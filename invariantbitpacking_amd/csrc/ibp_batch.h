@@ -57,6 +57,10 @@ BatchConfig adapt_config(const BatchConfig &base, int64_t total_rows, int num_cu
 int64_t fill_tasks(const ibp_fetch_task *tasks, int n, const BatchConfig &cfg, int max_blocks, TaskDev *out,
                    BatchShape *shape);
 
+// The plain fetch's launcher (ibp_fetch.hip): one table, parameters by value.
+int fetch_launch(const FetchParams &p, int elem_bytes, int index_bytes, int impl, int comp_len, int nblks,
+                 int nthreads, hipStream_t stream, int layout, bool host_resident);
+
 // Launches the one-thread kernel that zeroes the work-queue head on `s` (also
 // used to make the runtime set up a stream's hardware queue ahead of time).
 int reset_queue_head(unsigned long long *head, hipStream_t s);

@@ -220,10 +220,20 @@ struct ibp_server {
         BatchShape shape;
         const int64_t items =
             fill_tasks(tasks.data(), (int)tasks.size(), cfg_now, device_info().num_cus * 16, sl.host_tasks, &shape);
-        const hipError_t e = hipMemcpyAsync(sl.dev_tasks, sl.host_tasks, sizeof(TaskDev) * tasks.size(),
-                                            hipMemcpyHostToDevice, sl.stream);
-        int st = e == hipSuccess ? launch_batch(sl.dev_tasks, (int)tasks.size(), items, sl.head, cfg_now, sl.stream, shape)
+        int st;
+        if (tasks.size() == 1) {
+            // One request: the plain fetch, its parameters passed by value
+            // (no task-table copy before the kernel).
+            const ibp_fetch_task &t = tasks[0];
+            st = fetch_launch(sl.host_tasks[0].p, t.elem_bytes, t.index_bytes, -1,
+                              t.comp_len > 0 ? t.comp_len : (int)t.vec_size, 0, 0, sl.stream, kLayoutSlots,
+                              t.comp_loc == IBP_LOC_HOST);
+        } else {
+            const hipError_t e = hipMemcpyAsync(sl.dev_tasks, sl.host_tasks, sizeof(TaskDev) * tasks.size(),
+                                                hipMemcpyHostToDevice, sl.stream);
+            st = e == hipSuccess ? launch_batch(sl.dev_tasks, (int)tasks.size(), items, sl.head, cfg_now, sl.stream, shape)
                                  : IBP_ERR_HIP;
+        }
         if (st == IBP_OK && hipEventRecord(sl.done, sl.stream) != hipSuccess) st = IBP_ERR_HIP;
         if (st != IBP_OK) worker_error = "launch failed";
         const double t_issued = now_us();
