@@ -257,15 +257,29 @@ int ibp_server_register(ibp_server *srv, int elem_bytes, const void *comp, int64
                         const int32_t *bitmask, int comp_len, int comp_loc, int *set_id);
 /* Submits a request: fetch rows `index` (int64, host memory, copied at submit)
  * of set `set_id`.  Returns immediately with *req_id; the request is admitted
- * when its output fits the arena. */
+ * when its output fits the arena.  Requests are admitted in submission order. */
 int ibp_server_submit(ibp_server *srv, int set_id, const int64_t *index, int64_t n,
                       int64_t *req_id);
+/* ibp_server_submit with the request's arrival time (ibp_server_clock_us
+ * microseconds, not in the future; <= 0 means now): latencies and the
+ * server's p50/p99 count from it, e.g. from a request's scheduled arrival. */
+int ibp_server_submit_at(ibp_server *srv, int set_id, const int64_t *index, int64_t n,
+                         double arrival_us, int64_t *req_id);
+/* The server's clock: host steady-clock microseconds. */
+double ibp_server_clock_us(void);
 /* Waits for a request (timeout_ms < 0: forever).  On success *out points to
  * the [n x vec_size] result in the device arena, valid until
  * ibp_server_release(req_id); *latency_us is arrival -> completion. */
 int ibp_server_wait(ibp_server *srv, int64_t req_id, int timeout_ms, void **out,
                     double *latency_us);
+/* Returns a finished request's output region to the arena at once: the
+ * caller must have no work in flight that still reads it. */
 int ibp_server_release(ibp_server *srv, int64_t req_id);
+/* Returns the region once the work queued on `stream` (hipStream_t) before
+ * this call has finished: an event is recorded on `stream`, and later requests
+ * are placed in the region only after it completes.  The request id is
+ * invalid on return. */
+int ibp_server_release_after(ibp_server *srv, int64_t req_id, void *stream);
 /* Statistics since creation: completed requests, launches, rows, p50/p99
  * latency (us), peak arena use (bytes), admissions that had to wait. */
 typedef struct ibp_server_stats {
@@ -300,6 +314,13 @@ int ibp_read_file(const char *path, void *dst, size_t capacity, int threads, siz
  * bit-by-bit model on `trials` random cases; returns IBP_OK or IBP_ERR_INVALID
  * with the first mismatch in ibp_last_error(). */
 int ibp_selftest_bitops(int trials, uint64_t seed);
+/* Runs the server's admission bookkeeping (output arena, pinned index arena,
+ * in-order admission) on the host with a mock launcher: a hand-built case in
+ * which the queue head can only be admitted by taking back the pinned index
+ * space of later requests, then `trials` random submit / launch / complete /
+ * release sequences on small arenas, each of which must drain with every
+ * request's indices intact.  No GPU work. */
+int ibp_selftest_serve_admission(int trials, uint64_t seed);
 
 #ifdef __cplusplus
 }

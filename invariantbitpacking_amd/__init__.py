@@ -56,6 +56,15 @@ def _loc(t):
     return 1 if t.is_cuda else 2
 
 
+def _kernel_readable(orig, t):
+    """`t` converted from `orig` (dtype or layout): a host copy of a pinned
+    tensor is pinned again, since the kernels read host arguments in place
+    and a pageable copy is not visible to the device."""
+    if t is not orig and not t.is_cuda and not t.is_pinned():
+        t = t.pin_memory()
+    return t
+
+
 def _index_arg(index_arr, require_cuda):
     """Returns (tensor kept alive, element bytes) for an index array."""
     if index_arr is None:
@@ -68,8 +77,7 @@ def _index_arg(index_arr, require_cuda):
     idx = index_arr
     if idx.dtype not in (torch.int32, torch.int64):
         idx = idx.to(torch.int64)
-    idx = idx.contiguous()
-    return idx, idx.element_size()
+    return _kernel_readable(index_arr, idx.contiguous()), idx.element_size()
 
 
 def _clusters_arg(clusters, mask, rows):
@@ -80,7 +88,7 @@ def _clusters_arg(clusters, mask, rows):
     _check(clusters.dim() == 1 and clusters.numel() >= rows, "Clusters must be 1D with one entry per row")
     _check(mask.dim() == 2, "With clusters, mask and bitval must be 2D [num_clusters x vec_size]")
     cl = clusters if clusters.dtype == torch.int32 else clusters.to(torch.int32)
-    return cl.contiguous(), mask.size(0)
+    return _kernel_readable(clusters, cl.contiguous()), mask.size(0)
 
 
 def _layout(offsets=None, stream_bytes=0, clusters=None, num_clusters=1):

@@ -17,6 +17,7 @@
 #include <deque>
 #include <map>
 #include <mutex>
+#include <random>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -54,7 +55,7 @@ enum class St { Pending, Admitted, Running, Done };
 
 struct Req {
     int set = 0;
-    std::vector<int64_t> index; // only while no pinned index space was free
+    std::vector<int64_t> index; // only while the request holds no pinned index space
     int64_t n = 0;
     size_t out_off = 0, out_bytes = 0;
     size_t idx_off = 0;
@@ -68,16 +69,21 @@ struct Req {
 struct Arena {
     size_t size = 0, used = 0, peak = 0;
     std::map<size_t, size_t> free_list; // offset -> length
+    static size_t granules(size_t bytes)
+    {
+        bytes = (bytes + 255) / 256 * 256;
+        return bytes == 0 ? 256 : bytes;
+    }
     void init(size_t s)
     {
         size = s;
+        used = peak = 0;
         free_list.clear();
         free_list[0] = s;
     }
     bool alloc(size_t bytes, size_t cap, size_t *off)
     {
-        bytes = (bytes + 255) / 256 * 256;
-        if (bytes == 0) bytes = 256;
+        bytes = granules(bytes);
         if (used + bytes > cap) return false;
         for (auto it = free_list.begin(); it != free_list.end(); ++it) {
             if (it->second >= bytes) {
@@ -95,8 +101,7 @@ struct Arena {
     }
     void release(size_t off, size_t bytes)
     {
-        bytes = (bytes + 255) / 256 * 256;
-        if (bytes == 0) bytes = 256;
+        bytes = granules(bytes);
         used -= bytes;
         auto it = free_list.emplace(off, bytes).first;
         auto next = std::next(it);
@@ -112,6 +117,94 @@ struct Arena {
             }
         }
     }
+};
+
+// Request bookkeeping and admission, free of HIP calls (the host self-test
+// drives it with a mock launcher).  Requests are admitted in arrival order
+// when their output fits the output arena and their indices are in the pinned
+// index arena, where the kernel reads them in place.  A request takes pinned
+// index space at submit when some is free; one that finds none keeps its
+// indices in host memory and is copied in when it reaches the head of the
+// queue.  The head can then always be admitted once the running requests
+// finish: if the pinned arena is too full for it, the pending requests behind
+// it give their pinned space back (their indices are copied out again), since
+// they cannot be admitted before it anyway.
+struct Admission {
+    Arena out, pin;
+    char *pin_mem = nullptr;
+    size_t budget_override = 0;
+    std::unordered_map<int64_t, Req> reqs;
+    std::deque<int64_t> pending, admitted;
+    int64_t waited = 0, reclaimed = 0;
+
+    size_t out_cap() const { return budget_override ? std::min(budget_override, out.size) : out.size; }
+
+    // Pinned index space for the head request `r`, reclaiming the space of the
+    // later pending requests (newest first) while it does not fit.
+    bool pin_head(Req &r, size_t *off)
+    {
+        const size_t bytes = sizeof(int64_t) * (size_t)r.n;
+        if (pin.alloc(bytes, pin.size, off)) return true;
+        for (size_t q = pending.size(); q-- > 1;) {
+            Req &o = reqs[pending[q]];
+            if (!o.idx_pinned) continue;
+            const int64_t *src = (const int64_t *)(pin_mem + o.idx_off);
+            o.index.assign(src, src + o.n);
+            pin.release(o.idx_off, sizeof(int64_t) * (size_t)o.n);
+            o.idx_pinned = false;
+            ++reclaimed;
+            if (pin.alloc(bytes, pin.size, off)) return true;
+        }
+        return false;
+    }
+
+    // Admits pending requests in order while they fit.
+    void admit(double now)
+    {
+        while (!pending.empty()) {
+            Req &r = reqs[pending.front()];
+            size_t off = 0, ioff = r.idx_off;
+            bool ok = out.alloc(r.out_bytes, out_cap(), &off);
+            if (ok && !r.idx_pinned && !pin_head(r, &ioff)) {
+                out.release(off, r.out_bytes);
+                ok = false;
+            }
+            if (!ok) {
+                if (!r.waited) {
+                    r.waited = true;
+                    ++waited;
+                }
+                return; // FIFO: later requests wait behind the head
+            }
+            if (!r.idx_pinned) {
+                memcpy(pin_mem + ioff, r.index.data(), sizeof(int64_t) * (size_t)r.n);
+                std::vector<int64_t>().swap(r.index);
+                r.idx_off = ioff;
+                r.idx_pinned = true;
+            }
+            r.out_off = off;
+            r.st = St::Admitted;
+            r.t_admit = now;
+            admitted.push_back(pending.front());
+            pending.pop_front();
+        }
+    }
+
+    // A launched request's kernel has finished: its pinned indices are free.
+    void complete(Req &r, double now)
+    {
+        pin.release(r.idx_off, sizeof(int64_t) * (size_t)r.n);
+        r.idx_pinned = false;
+        r.st = St::Done;
+        r.t_done = now;
+    }
+};
+
+// An output region released with a stream: freed once the work the caller
+// queued on that stream before the release has finished.
+struct DeferredFree {
+    size_t off, bytes;
+    hipEvent_t ev;
 };
 
 // One launch slot: a stream and the batch's task table, staged in pinned host
@@ -132,55 +225,26 @@ struct ibp_server {
     ibp_server_config cfg{};
     int device = 0;
     char *arena_mem = nullptr;
-    Arena arena;
     char *pin_mem = nullptr; // pinned host memory of request indices
-    Arena pin_arena;
-    size_t budget_override = 0;
+    Admission adm;
     BatchConfig bcfg{256, 4, 8, 0, 1, 0, IBP_SERVE_SEGMENTED};
     std::vector<Set> sets;
     std::vector<Slot> slots;
-    std::unordered_map<int64_t, Req> reqs;
-    std::deque<int64_t> pending, admitted;
+    std::vector<DeferredFree> deferred;
+    std::vector<hipEvent_t> event_pool;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::thread worker;
     bool stop = false;
     int64_t next_id = 1;
-    int64_t completed = 0, launches = 0, rows = 0, launched_reqs = 0, waited = 0;
+    int64_t completed = 0, launches = 0, rows = 0, launched_reqs = 0;
     std::vector<double> latencies;
     std::string worker_error;
 
-    void admit_locked()
+    void fail_locked(const char *what, hipError_t e)
     {
-        const size_t cap = budget_override ? std::min(budget_override, arena.size) : arena.size;
-        while (!pending.empty()) {
-            Req &r = reqs[pending.front()];
-            size_t off = 0, ioff = 0;
-            const bool need_pin = !r.idx_pinned;
-            bool ok = !need_pin || pin_arena.alloc(sizeof(int64_t) * (size_t)r.n, pin_arena.size, &ioff);
-            if (ok && !arena.alloc(r.out_bytes, cap, &off)) {
-                if (need_pin) pin_arena.release(ioff, sizeof(int64_t) * (size_t)r.n);
-                ok = false;
-            }
-            if (!ok) {
-                if (!r.waited) {
-                    r.waited = true;
-                    ++waited;
-                }
-                return; // FIFO: later requests wait behind the head
-            }
-            if (need_pin) {
-                memcpy(pin_mem + ioff, r.index.data(), sizeof(int64_t) * (size_t)r.n);
-                std::vector<int64_t>().swap(r.index);
-                r.idx_off = ioff;
-                r.idx_pinned = true;
-            }
-            r.out_off = off;
-            r.st = St::Admitted;
-            r.t_admit = now_us();
-            admitted.push_back(pending.front());
-            pending.pop_front();
-        }
+        if (worker_error.empty()) worker_error = std::string(what) + ": " + hipGetErrorString(e);
+        cv_done.notify_all();
     }
 
     bool launch_locked(Slot &sl)
@@ -189,8 +253,8 @@ struct ibp_server {
         std::vector<ibp_fetch_task> tasks;
         int64_t nrows = 0;
         sl.reqs.clear();
-        while (!admitted.empty() && (int)sl.reqs.size() < cfg.max_batch_requests) {
-            Req &r = reqs[admitted.front()];
+        while (!adm.admitted.empty() && (int)sl.reqs.size() < cfg.max_batch_requests) {
+            Req &r = adm.reqs[adm.admitted.front()];
             if (!sl.reqs.empty() && nrows + r.n > cfg.max_batch_rows) break;
             const Set &s = sets[r.set];
             ibp_fetch_task t;
@@ -212,8 +276,8 @@ struct ibp_server {
             tasks.push_back(t);
             nrows += r.n;
             r.st = St::Running;
-            sl.reqs.push_back(admitted.front());
-            admitted.pop_front();
+            sl.reqs.push_back(adm.admitted.front());
+            adm.admitted.pop_front();
         }
         if (sl.reqs.empty()) return false;
         const BatchConfig cfg_now = adapt_config(bcfg, nrows, device_info().num_cus);
@@ -234,12 +298,18 @@ struct ibp_server {
             st = e == hipSuccess ? launch_batch(sl.dev_tasks, (int)tasks.size(), items, sl.head, cfg_now, sl.stream, shape)
                                  : IBP_ERR_HIP;
         }
-        if (st == IBP_OK && hipEventRecord(sl.done, sl.stream) != hipSuccess) st = IBP_ERR_HIP;
-        if (st != IBP_OK) worker_error = "launch failed";
+        hipError_t rec = hipSuccess;
+        if (st == IBP_OK) rec = hipEventRecord(sl.done, sl.stream);
+        if (st != IBP_OK) {
+            if (worker_error.empty()) worker_error = std::string("launch failed: ") + ibp_last_error();
+            cv_done.notify_all();
+        } else if (rec != hipSuccess) {
+            fail_locked("hipEventRecord", rec);
+        }
         const double t_issued = now_us();
         for (int64_t id : sl.reqs) {
-            reqs[id].t_launch = t_launch;
-            reqs[id].t_issued = t_issued;
+            adm.reqs[id].t_launch = t_launch;
+            adm.reqs[id].t_issued = t_issued;
         }
         sl.busy = true;
         ++launches;
@@ -248,19 +318,19 @@ struct ibp_server {
         return true;
     }
 
+    // Completed launches: their requests are done.  A failed launch or a
+    // faulted kernel stops the server: every wait then reports the error.
     void poll_locked()
     {
         for (Slot &sl : slots) {
             if (!sl.busy) continue;
             const hipError_t q = hipEventQuery(sl.done);
             if (q == hipErrorNotReady) continue;
+            if (q != hipSuccess) fail_locked("fetch kernel", q);
             const double t = now_us();
             for (int64_t id : sl.reqs) {
-                Req &r = reqs[id];
-                pin_arena.release(r.idx_off, sizeof(int64_t) * (size_t)r.n);
-                r.idx_pinned = false;
-                r.st = St::Done;
-                r.t_done = t;
+                Req &r = adm.reqs[id];
+                adm.complete(r, t);
                 latencies.push_back(r.t_done - r.t_arrival);
                 ++completed;
             }
@@ -270,18 +340,40 @@ struct ibp_server {
         }
     }
 
+    // Output regions released with a stream are returned to the arena once
+    // the caller's work queued before the release has finished.
+    void poll_deferred_locked()
+    {
+        for (size_t i = 0; i < deferred.size();) {
+            const hipError_t q = hipEventQuery(deferred[i].ev);
+            if (q == hipErrorNotReady) {
+                ++i;
+                continue;
+            }
+            if (q != hipSuccess) fail_locked("caller stream", q);
+            adm.out.release(deferred[i].off, deferred[i].bytes);
+            event_pool.push_back(deferred[i].ev);
+            deferred[i] = deferred.back();
+            deferred.pop_back();
+        }
+    }
+
     void run()
     {
         (void)hipSetDevice(device);
         std::unique_lock<std::mutex> lk(mu);
         while (!stop) {
             poll_locked();
-            admit_locked();
+            poll_deferred_locked();
+            adm.admit(now_us());
             for (Slot &sl : slots)
-                if (!sl.busy && !admitted.empty()) launch_locked(sl);
+                if (!sl.busy && !adm.admitted.empty() && worker_error.empty()) launch_locked(sl);
             bool busy = false;
             for (Slot &sl : slots) busy = busy || sl.busy;
-            if (!busy && admitted.empty()) {
+            // Idle unless a launch is in flight, or a pending request waits
+            // for a region whose release is still in flight on a stream.
+            const bool deferred_wait = !deferred.empty() && !adm.pending.empty();
+            if (!busy && !deferred_wait && (adm.admitted.empty() || !worker_error.empty())) {
                 cv_work.wait(lk);
             } else {
                 lk.unlock();
@@ -309,11 +401,14 @@ extern "C" int ibp_server_create(const ibp_server_config *cfg, ibp_server **out)
     hipError_t e = hipMalloc((void **)&srv->arena_mem, cfg->hbm_budget_bytes);
     if (e != hipSuccess) return fail("arena allocation", e);
     if ((e = hipDeviceSynchronize()) != hipSuccess) return fail("setup", e);
-    srv->arena.init(cfg->hbm_budget_bytes);
+    srv->adm.out.init(cfg->hbm_budget_bytes);
+    // Room for four requests of max_batch_rows indices (at least 64 MiB):
+    // any single request fits once the running ones have finished.
     size_t pin_bytes = sizeof(int64_t) * (size_t)cfg->max_batch_rows * 4;
     if (pin_bytes < ((size_t)64 << 20)) pin_bytes = (size_t)64 << 20;
     if ((e = hipHostMalloc((void **)&srv->pin_mem, pin_bytes, 0)) != hipSuccess) return fail("pinned index arena", e);
-    srv->pin_arena.init(pin_bytes);
+    srv->adm.pin.init(pin_bytes);
+    srv->adm.pin_mem = srv->pin_mem;
     srv->slots.resize((size_t)cfg->num_streams);
     for (Slot &sl : srv->slots) {
         // Streams keep the default priority.
@@ -354,6 +449,12 @@ extern "C" int ibp_server_destroy(ibp_server *srv)
         if (sl.host_tasks) (void)hipHostFree(sl.host_tasks);
         if (sl.head) (void)hipFree(sl.head);
     }
+    // Regions released with a stream: the caller's reads of the arena finish first.
+    for (DeferredFree &d : srv->deferred) {
+        (void)hipEventSynchronize(d.ev);
+        (void)hipEventDestroy(d.ev);
+    }
+    for (hipEvent_t ev : srv->event_pool) (void)hipEventDestroy(ev);
     if (srv->arena_mem) (void)hipFree(srv->arena_mem);
     if (srv->pin_mem) (void)hipHostFree(srv->pin_mem);
     delete srv;
@@ -375,61 +476,101 @@ extern "C" int ibp_server_register(ibp_server *srv, int elem_bytes, const void *
     return IBP_OK;
 }
 
-extern "C" int ibp_server_submit(ibp_server *srv, int set_id, const int64_t *index, int64_t n, int64_t *req_id)
+extern "C" double ibp_server_clock_us(void) { return now_us(); }
+
+extern "C" int ibp_server_submit_at(ibp_server *srv, int set_id, const int64_t *index, int64_t n, double arrival_us,
+                                    int64_t *req_id)
 {
-    const double t_arrival = now_us(); // latency includes copying the index
+    const double t_call = now_us(); // latency includes copying the index
     IBP_CHECK(srv && req_id, "null argument");
     IBP_CHECK(n >= 1 && index != nullptr, "a request needs at least one index");
+    IBP_CHECK(arrival_us <= t_call, "arrival time %.1f us is in the future (now %.1f us)", arrival_us, t_call);
     std::unique_lock<std::mutex> lk(srv->mu);
     IBP_CHECK(set_id >= 0 && set_id < (int)srv->sets.size(), "unknown set %d", set_id);
     IBP_CHECK(n <= srv->cfg.max_batch_rows, "request of %lld rows exceeds max_batch_rows (%lld)", (long long)n,
               (long long)srv->cfg.max_batch_rows);
     const Set &s = srv->sets[set_id];
     const size_t bytes = (size_t)n * (size_t)s.vec_size * (size_t)s.elem_bytes;
-    if (bytes > srv->arena.size)
+    if (bytes > srv->adm.out.size)
         return set_error(IBP_ERR_NOMEM, "request output (%zu bytes) can never fit the arena (%zu bytes)", bytes,
-                         srv->arena.size);
+                         srv->adm.out.size);
     const int64_t id = srv->next_id++;
     size_t ioff = 0;
-    const bool pinned = srv->pin_arena.alloc(sizeof(int64_t) * (size_t)n, srv->pin_arena.size, &ioff);
+    const bool pinned = srv->adm.pin.alloc(sizeof(int64_t) * (size_t)n, srv->adm.pin.size, &ioff);
     lk.unlock();
     // The copy runs outside the lock; the request is not queued until it is done.
     std::vector<int64_t> vec;
     if (pinned) memcpy(srv->pin_mem + ioff, index, sizeof(int64_t) * (size_t)n);
     else vec.assign(index, index + n);
     lk.lock();
-    Req &r = srv->reqs[id];
+    Req &r = srv->adm.reqs[id];
     r.set = set_id;
     r.index.swap(vec);
     r.idx_pinned = pinned;
     r.idx_off = ioff;
     r.n = n;
     r.out_bytes = bytes;
-    r.t_arrival = t_arrival;
-    srv->pending.push_back(id);
+    r.t_arrival = arrival_us > 0 ? arrival_us : t_call;
+    srv->adm.pending.push_back(id);
     lk.unlock();
     srv->cv_work.notify_all();
     *req_id = id;
     return IBP_OK;
 }
 
+extern "C" int ibp_server_submit(ibp_server *srv, int set_id, const int64_t *index, int64_t n, int64_t *req_id)
+{
+    return ibp_server_submit_at(srv, set_id, index, n, 0.0, req_id);
+}
+
 extern "C" int ibp_server_wait(ibp_server *srv, int64_t req_id, int timeout_ms, void **out, double *latency_us)
 {
     IBP_CHECK(srv, "null server");
     std::unique_lock<std::mutex> lk(srv->mu);
-    auto it = srv->reqs.find(req_id);
-    IBP_CHECK(it != srv->reqs.end(), "unknown request %lld", (long long)req_id);
-    // std::map iterators stay valid while other requests come and go.
-    auto ready = [&] { return it->second.st == St::Done || !srv->worker_error.empty(); };
+    IBP_CHECK(srv->adm.reqs.count(req_id) != 0, "unknown request %lld", (long long)req_id);
+    // Looked up again after every wake-up: submits may rehash the table.
+    auto ready = [&] {
+        auto it = srv->adm.reqs.find(req_id);
+        return it == srv->adm.reqs.end() || it->second.st == St::Done || !srv->worker_error.empty();
+    };
     if (timeout_ms < 0) {
         srv->cv_done.wait(lk, ready);
     } else if (!srv->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready)) {
         return set_error(IBP_ERR_BUSY, "request %lld not finished", (long long)req_id);
     }
     if (!srv->worker_error.empty()) return set_error(IBP_ERR_HIP, "server worker: %s", srv->worker_error.c_str());
+    auto it = srv->adm.reqs.find(req_id);
+    IBP_CHECK(it != srv->adm.reqs.end(), "request %lld was released while waited for", (long long)req_id);
     const Req &r = it->second;
     if (out) *out = srv->arena_mem + r.out_off;
     if (latency_us) *latency_us = r.t_done - r.t_arrival;
+    return IBP_OK;
+}
+
+extern "C" int ibp_server_release_after(ibp_server *srv, int64_t req_id, void *stream)
+{
+    IBP_CHECK(srv, "null server");
+    {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        auto it = srv->adm.reqs.find(req_id);
+        IBP_CHECK(it != srv->adm.reqs.end(), "unknown request %lld", (long long)req_id);
+        IBP_CHECK(it->second.st == St::Done, "request %lld is still in flight", (long long)req_id);
+        hipEvent_t ev = nullptr;
+        if (!srv->event_pool.empty()) {
+            ev = srv->event_pool.back();
+            srv->event_pool.pop_back();
+        } else {
+            IBP_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        }
+        const hipError_t e = hipEventRecord(ev, as_stream(stream));
+        if (e != hipSuccess) {
+            srv->event_pool.push_back(ev);
+            return set_error(IBP_ERR_HIP, "hipEventRecord on the caller's stream: %s", hipGetErrorString(e));
+        }
+        srv->deferred.push_back(DeferredFree{it->second.out_off, it->second.out_bytes, ev});
+        srv->adm.reqs.erase(it);
+    }
+    srv->cv_work.notify_all();
     return IBP_OK;
 }
 
@@ -438,11 +579,11 @@ extern "C" int ibp_server_release(ibp_server *srv, int64_t req_id)
     IBP_CHECK(srv, "null server");
     {
         std::lock_guard<std::mutex> lk(srv->mu);
-        auto it = srv->reqs.find(req_id);
-        IBP_CHECK(it != srv->reqs.end(), "unknown request %lld", (long long)req_id);
+        auto it = srv->adm.reqs.find(req_id);
+        IBP_CHECK(it != srv->adm.reqs.end(), "unknown request %lld", (long long)req_id);
         IBP_CHECK(it->second.st == St::Done, "request %lld is still in flight", (long long)req_id);
-        srv->arena.release(it->second.out_off, it->second.out_bytes);
-        srv->reqs.erase(it);
+        srv->adm.out.release(it->second.out_off, it->second.out_bytes);
+        srv->adm.reqs.erase(it);
     }
     srv->cv_work.notify_all();
     return IBP_OK;
@@ -461,8 +602,8 @@ extern "C" int ibp_server_stats_get(ibp_server *srv, ibp_server_stats *st)
     st->p50_us = pct(0.50);
     st->p99_us = pct(0.99);
     st->mean_batch_requests = srv->launches ? (double)srv->launched_reqs / (double)srv->launches : 0.0;
-    st->peak_arena_bytes = srv->arena.peak;
-    st->waited_admissions = srv->waited;
+    st->peak_arena_bytes = srv->adm.out.peak;
+    st->waited_admissions = srv->adm.waited;
     return IBP_OK;
 }
 
@@ -471,7 +612,7 @@ extern "C" int ibp_server_set_budget_override(ibp_server *srv, size_t bytes)
     IBP_CHECK(srv, "null server");
     {
         std::lock_guard<std::mutex> lk(srv->mu);
-        srv->budget_override = bytes;
+        srv->adm.budget_override = bytes;
     }
     srv->cv_work.notify_all();
     return IBP_OK;
@@ -481,8 +622,8 @@ extern "C" int ibp_server_request_times(ibp_server *srv, int64_t req_id, double 
 {
     IBP_CHECK(srv && times, "null argument");
     std::lock_guard<std::mutex> lk(srv->mu);
-    auto it = srv->reqs.find(req_id);
-    IBP_CHECK(it != srv->reqs.end(), "unknown request %lld", (long long)req_id);
+    auto it = srv->adm.reqs.find(req_id);
+    IBP_CHECK(it != srv->adm.reqs.end(), "unknown request %lld", (long long)req_id);
     const Req &r = it->second;
     times[0] = r.t_arrival;
     times[1] = r.t_admit;
@@ -490,4 +631,154 @@ extern "C" int ibp_server_request_times(ibp_server *srv, int64_t req_id, double 
     times[3] = r.t_issued;
     times[4] = r.t_done;
     return IBP_OK;
+}
+
+// ---- host self-test of the admission logic ----------------------------------------
+
+namespace {
+
+// Checks that a request's indices reached the pinned arena intact.
+bool indices_intact(const Admission &a, int64_t id, const Req &r)
+{
+    const int64_t *p = (const int64_t *)(a.pin_mem + r.idx_off);
+    for (int64_t k = 0; k < r.n; ++k)
+        if (p[k] != id * 1000003 + k) return false;
+    return true;
+}
+
+// Submits like ibp_server_submit: pinned space at once when some is free.
+void mock_submit(Admission &a, int64_t id, int64_t n, size_t out_bytes)
+{
+    std::vector<int64_t> idx((size_t)n);
+    for (int64_t k = 0; k < n; ++k) idx[(size_t)k] = id * 1000003 + k;
+    Req &r = a.reqs[id];
+    size_t off = 0;
+    r.n = n;
+    r.out_bytes = out_bytes;
+    if (a.pin.alloc(sizeof(int64_t) * (size_t)n, a.pin.size, &off)) {
+        memcpy(a.pin_mem + off, idx.data(), sizeof(int64_t) * (size_t)n);
+        r.idx_pinned = true;
+        r.idx_off = off;
+    } else {
+        r.index.swap(idx);
+    }
+    a.pending.push_back(id);
+}
+
+// Mock launcher: admitted requests start running (their pinned indices are
+// checked), up to `max_running` at a time.
+int mock_launch(Admission &a, std::vector<int64_t> &running, size_t max_running)
+{
+    while (!a.admitted.empty() && running.size() < max_running) {
+        const int64_t id = a.admitted.front();
+        Req &r = a.reqs[id];
+        if (!r.idx_pinned || !indices_intact(a, id, r))
+            return set_error(IBP_ERR_INVALID, "request %lld launched without its indices", (long long)id);
+        r.st = St::Running;
+        running.push_back(id);
+        a.admitted.pop_front();
+    }
+    return IBP_OK;
+}
+
+int check_arenas_empty(const Admission &a)
+{
+    if (a.pin.used != 0 || a.out.used != 0)
+        return set_error(IBP_ERR_INVALID, "arenas not empty after the drain (pinned %zu, output %zu bytes)",
+                         a.pin.used, a.out.used);
+    return IBP_OK;
+}
+
+// The case a strictly FIFO queue without reclaim deadlocks on: the head has no
+// pinned indices, later requests took all the pinned space at submit, and the
+// running request's space alone is too little for the head.
+int admission_reclaim_case()
+{
+    std::vector<char> mem(8 * 256);
+    Admission a;
+    a.pin_mem = mem.data();
+    a.pin.init(mem.size());
+    a.out.init(1 << 20);
+    mock_submit(a, 1, 4 * 32, 4096); // runs, holds 4 of 8 pinned granules
+    a.admit(0);
+    std::vector<int64_t> running;
+    int st = mock_launch(a, running, 1);
+    if (st != IBP_OK) return st;
+    mock_submit(a, 2, 6 * 32, 4096); // needs 6 granules: none at submit
+    mock_submit(a, 3, 2 * 32, 4096); // takes 2
+    mock_submit(a, 4, 2 * 32, 4096); // takes 2: pinned space full
+    if (a.reqs[2].idx_pinned || !a.reqs[3].idx_pinned || !a.reqs[4].idx_pinned)
+        return set_error(IBP_ERR_INVALID, "reclaim case: unexpected submit-time pinning");
+    a.complete(a.reqs[1], 0); // frees 4 granules; the head needs 6
+    running.clear();
+    a.out.release(a.reqs[1].out_off, a.reqs[1].out_bytes);
+    a.reqs.erase(1);
+    a.admit(0);
+    if (a.admitted.empty() || a.admitted.front() != 2)
+        return set_error(IBP_ERR_INVALID, "reclaim case: the head request was not admitted");
+    while (!a.admitted.empty() || !a.pending.empty()) {
+        st = mock_launch(a, running, 1);
+        if (st != IBP_OK) return st;
+        for (int64_t id : running) {
+            a.complete(a.reqs[id], 0);
+            a.out.release(a.reqs[id].out_off, a.reqs[id].out_bytes);
+            a.reqs.erase(id);
+        }
+        running.clear();
+        a.admit(0);
+    }
+    return check_arenas_empty(a);
+}
+
+// Random submits, launches, completions and releases on small arenas; after
+// the last submit everything must drain (no admission deadlock) with every
+// request's indices intact.
+int admission_random_case(std::mt19937_64 &rng, int steps)
+{
+    const size_t pin_granules = 4 + rng() % 12;
+    std::vector<char> mem(pin_granules * 256);
+    Admission a;
+    a.pin_mem = mem.data();
+    a.pin.init(mem.size());
+    a.out.init((8 + rng() % 24) * 256);
+    const int64_t max_n = (int64_t)pin_granules * 32; // a request fits the empty pinned arena
+    const size_t max_out = a.out.size;
+    std::vector<int64_t> running, done;
+    int64_t next = 1;
+    for (int step = 0; step < steps || !a.pending.empty() || !a.admitted.empty() || !running.empty() || !done.empty();
+         ++step) {
+        if (step > steps + 100000) return set_error(IBP_ERR_INVALID, "admission did not drain");
+        const int op = (int)(rng() % 4);
+        if (op == 0 && step < steps) {
+            const int64_t n = 1 + (int64_t)(rng() % (uint64_t)max_n);
+            mock_submit(a, next++, n, 1 + rng() % max_out);
+        } else if (op == 1 && !running.empty()) { // a launch finishes
+            const size_t k = rng() % running.size();
+            a.complete(a.reqs[running[k]], 0);
+            done.push_back(running[k]);
+            running[k] = running.back();
+            running.pop_back();
+        } else if (op == 2 && !done.empty()) { // the caller releases an output
+            const size_t k = rng() % done.size();
+            a.out.release(a.reqs[done[k]].out_off, a.reqs[done[k]].out_bytes);
+            a.reqs.erase(done[k]);
+            done[k] = done.back();
+            done.pop_back();
+        }
+        a.admit(0);
+        const int st = mock_launch(a, running, 1 + rng() % 4);
+        if (st != IBP_OK) return st;
+    }
+    return check_arenas_empty(a);
+}
+
+} // namespace
+
+extern "C" int ibp_selftest_serve_admission(int trials, uint64_t seed)
+{
+    clear_error();
+    int st = admission_reclaim_case();
+    std::mt19937_64 rng(seed);
+    for (int t = 0; st == IBP_OK && t < trials; ++t) st = admission_random_case(rng, 200);
+    return st;
 }

@@ -43,11 +43,20 @@ class Server:
         self._sets[sid.value] = (v, comp.dtype)
         return sid.value
 
-    def submit(self, set_id, index):
-        """index: host int64 array / CPU tensor of row ids; returns a request id."""
+    @staticmethod
+    def clock_us():
+        """The server's clock (host steady-clock microseconds)."""
+        return _capi.lib().ibp_server_clock_us()
+
+    def submit(self, set_id, index, arrival_us=None):
+        """index: host int64 array / CPU tensor of row ids; returns a request id.
+        arrival_us (clock_us() time, not in the future): when the request
+        arrived, e.g. its scheduled arrival; latency counts from it (default:
+        now)."""
         idx = np.ascontiguousarray(np.asarray(index.cpu() if torch.is_tensor(index) else index, dtype=np.int64))
         rid = ctypes.c_int64(0)
-        _capi.call("ibp_server_submit", self._srv, set_id, idx.ctypes.data, idx.size, ctypes.byref(rid))
+        _capi.call("ibp_server_submit_at", self._srv, set_id, idx.ctypes.data, idx.size,
+                   0.0 if arrival_us is None else float(arrival_us), ctypes.byref(rid))
         return rid.value, set_id, idx.size
 
     def wait(self, req, timeout_ms=-1, view=True):
@@ -71,8 +80,14 @@ class Server:
         _capi.call("ibp_server_request_times", self._srv, req[0], t)
         return tuple(t)
 
-    def release(self, req):
-        _capi.call("ibp_server_release", self._srv, req[0])
+    def release(self, req, stream=None):
+        """Gives the request's output region back to the server.  The region is
+        reused only after the work queued so far on `stream` (default: the
+        current stream) has finished, so reads of the view queued before the
+        release stay valid.  Work on other streams that reads the view must be
+        finished (or ordered before `stream`) by the caller."""
+        s = torch.cuda.current_stream() if stream is None else stream
+        _capi.call("ibp_server_release_after", self._srv, req[0], ctypes.c_void_p(s.cuda_stream))
 
     def stats(self):
         st = _capi.ServerStats()

@@ -42,6 +42,13 @@ def test_selftest_bitops_host(capi):
     assert capi.lib().ibp_selftest_bitops(50000, 12345) == 0, capi.last_error()
 
 
+def test_selftest_serve_admission_host(capi):
+    """The server's admission bookkeeping with a mock launcher: the queue head
+    is admitted by taking back later requests' pinned index space, and random
+    submit / launch / complete / release sequences always drain."""
+    assert capi.lib().ibp_selftest_serve_admission(300, 7) == 0, capi.last_error()
+
+
 def test_invalid_arguments_are_status_codes(capi):
     L = capi.lib()
     st = L.ibp_decompress_fetch(3, None, None, 10, 4, None, None, None, None, 0, 10, 0, 0, 0, -1, 0, None)
