@@ -21,6 +21,11 @@ DeviceInfo g_info[64];
 unsigned long long *g_err_counter[64] = {nullptr};
 } // namespace
 
+// The index-error counter is a device variable of this code object, so no call
+// allocates, clears or synchronises for it (a fetch may be the first call, and
+// may be under stream capture).  It is zero when the library is loaded.
+static __device__ unsigned long long err_counter;
+
 int set_error(int status, const char *fmt, ...)
 {
     char buf[1024];
@@ -76,10 +81,12 @@ unsigned long long *index_error_counter()
     if (g_err_counter[dev]) return g_err_counter[dev];
     std::lock_guard<std::mutex> lk(g_dev_mu);
     if (!g_err_counter[dev]) {
+        // Each device holds its own copy of err_counter at its own address.
         void *p = nullptr;
-        if (hipMalloc(&p, sizeof(unsigned long long)) != hipSuccess) return nullptr;
-        if (hipMemset(p, 0, sizeof(unsigned long long)) != hipSuccess) return nullptr;
-        if (hipDeviceSynchronize() != hipSuccess) return nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(err_counter)) != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
         g_err_counter[dev] = (unsigned long long *)p;
     }
     return g_err_counter[dev];
