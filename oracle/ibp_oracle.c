@@ -31,6 +31,7 @@
 #include "ibp_oracle.h"
 
 #include <limits.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -71,17 +72,32 @@ static inline int64_t align_up(int64_t a, int64_t w) { return (a + w - 1) / w * 
 
 /* ---- preprocess ---------------------------------------------------------- */
 
+void oracle_set_threads(int threads)
+{
+    if (threads > 0) omp_set_num_threads(threads);
+}
+
 void oracle_count_bits(int w, const void *data, int64_t num_vecs, int64_t vec_size, int32_t *cnt)
 {
     const int B = 8 * w;
     memset(cnt, 0, sizeof(int32_t) * (size_t)(vec_size * B));
-    /* ibp_preproc_kernels.cuh:19-28: every row, every column, every bit. */
-    for (int64_t i = 0; i < num_vecs; ++i)
-        for (int64_t j = 0; j < vec_size; ++j) {
-            uint64_t val = ld(data, i * vec_size + j, w);
-            for (int bit = 0; bit < B; ++bit)
-                if (val & (1ull << bit)) cnt[j * B + bit] += 1;
-        }
+    /* ibp_preproc_kernels.cuh:19-28: every row, every column, every bit.  Row
+     * blocks are counted per thread and the integer counts added: the sum
+     * does not depend on the order. */
+#pragma omp parallel
+    {
+        int32_t *part = (int32_t *)calloc((size_t)(vec_size * B), sizeof(int32_t));
+#pragma omp for schedule(static)
+        for (int64_t i = 0; i < num_vecs; ++i)
+            for (int64_t j = 0; j < vec_size; ++j) {
+                uint64_t val = ld(data, i * vec_size + j, w);
+                for (int bit = 0; bit < B; ++bit)
+                    if (val & (1ull << bit)) part[j * B + bit] += 1;
+            }
+#pragma omp critical
+        for (int64_t k = 0; k < vec_size * B; ++k) cnt[k] += part[k];
+        free(part);
+    }
 }
 
 void oracle_create_mask(int w, const int32_t *cnt, int64_t num_vecs, int64_t vec_size,
@@ -117,6 +133,7 @@ uint64_t oracle_check_feats(int w, const void *data, int64_t num_vecs, int64_t v
     uint64_t saved = 0;
     /* ibp_preproc_kernels.cuh:60-75: per row, bits of flagged elements; only
      * the excess over vec_size metadata bits counts. */
+#pragma omp parallel for schedule(static) reduction(+ : saved)
     for (int64_t i = 0; i < num_vecs; ++i) {
         uint64_t ctr = 0;
         for (int64_t j = 0; j < vec_size; ++j) {
@@ -489,6 +506,7 @@ static void km_assign(int w, const void *data, int64_t N, int64_t V, int K,
                       const uint64_t *centroids, int32_t *cluster)
 {
     const int B = 8 * w;
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < N; ++i) {
         int best = 0;
         int64_t best_d = INT64_MAX;
@@ -512,15 +530,16 @@ static void km_count(int w, const void *data, int64_t N, int64_t V, int K, const
     const int B = 8 * w;
     memset(count, 0, sizeof(int32_t) * (size_t)K);
     memset(bits_set, 0, sizeof(int32_t) * (size_t)((int64_t)K * V * B));
-    for (int64_t i = 0; i < N; ++i) {
-        int c = cluster[i];
-        count[c] += 1;
-        for (int64_t k = 0; k < V; ++k) {
+    for (int64_t i = 0; i < N; ++i) count[cluster[i]] += 1;
+    /* Columns are split over threads: each counter has one writer. */
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < V; ++k)
+        for (int64_t i = 0; i < N; ++i) {
+            const int c = cluster[i];
             uint64_t x = ld(data, i * V + k, w);
             for (int bit = 0; bit < B; ++bit)
                 if (x & (1ull << bit)) bits_set[((int64_t)c * V + k) * B + bit] += 1;
         }
-    }
 }
 
 void oracle_kmeans(int w, const void *data, int64_t N, int64_t V, int K, int64_t seed,

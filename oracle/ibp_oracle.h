@@ -20,6 +20,9 @@ extern "C" {
 
 /* include/preproc/ibp_preproc_kernels.cuh:8-31 (count_bit_kernel):
  * cnt[j*B + k] = number of rows whose column j has bit k set. */
+/* Threads of the OpenMP loops below (<= 0 leaves the OpenMP default). */
+void oracle_set_threads(int threads);
+
 void oracle_count_bits(int w, const void *data, int64_t num_vecs, int64_t vec_size,
                        int32_t *cnt);
 

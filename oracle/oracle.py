@@ -34,6 +34,7 @@ def lib():
         if not os.path.exists(_SO):
             build()
         L = ctypes.CDLL(_SO)
+        L.oracle_set_threads.argtypes = [ctypes.c_int]
         L.oracle_count_bits.argtypes = [ctypes.c_int, _P, _I64, _I64, _P]
         L.oracle_create_mask.argtypes = [ctypes.c_int, _P, _I64, _I64, ctypes.c_float, _P, _P]
         L.oracle_check_feats.argtypes = [ctypes.c_int, _P, _I64, _I64, _P, _P]
@@ -234,6 +235,11 @@ def decompress_condensed(stream, offsets, vec_size, mask, bitval, index=None, cl
                                       _ptr(_c(mask)), _ptr(_c(bitval)), _ptr(_clusters(clusters)), _ptr(idx), n,
                                       _ptr(out))
     return out
+
+
+def set_threads(n):
+    """Threads of the oracle's OpenMP loops (count, score, k-means, compress_mt)."""
+    lib().oracle_set_threads(int(n))
 
 
 def random_shuffle(n, seed=None):

@@ -407,3 +407,49 @@ def test_kmeans_empty_cluster_all_ones(oracle):
         if c not in used:
             assert np.all(m[c] == 0xFFFFFFFF) and np.all(b[c] == 0xFFFFFFFF)
     assert len(used) <= 2
+
+
+# Rows of the stride-32 OR-chain known answer: 3 rows of 33 int32 words, zero
+# except columns 0 and 1 (A = 0x0000FFFF, 16 bits; D = 0xFF000000, 8 bits).
+OR_CHAIN_ROWS = ((0x0000FFFF, 0), (0x0000FFFF, 0xFF000000), (0, 0xFF000000))
+
+
+def or_chain_data():
+    x = np.zeros((3, 33), dtype=np.uint32)
+    for r, (c0, c1) in enumerate(OR_CHAIN_ROWS):
+        x[r, 0], x[r, 1] = c0, c1
+    return x.view(np.int32)
+
+
+def or_chain_expected():
+    """Hand derivation (ibp_preproc_kmeans.cuh:220-236; host :150-233), K = 2.
+    srand(1): random_shuffle(0..2) = [0, 2, 1], so the initial centroids are
+    row 0 = (A, 0, .., 0) and row 2 = (0, D, 0, ..).  Hamming distances
+    (d0, d1): row0 (0, 24) row1 (8, 16) row2 (24, 0) -> clusters [0, 0, 1].
+    construct_bitmasks: lane 0 visits columns 0 and 32 with one accumulator,
+    so centroid 0 becomes (A, 0, .., 0, A) -- column 32 inherits column 0's
+    majority word -- and centroid 1 stays (0, D, 0, ..).  Next pass: row0
+    (16, 24) row1 (24, 16) row2 (40, 0) -> clusters [0, 1, 1], a fixed point
+    (centroid 0 = (A, .., A) from row 0 alone, centroid 1 = (0, D, ..) from
+    the bits both its rows share).  Without the chain column 32 stays 0 and
+    the clusters stay [0, 0, 1].  Final masks at threshold 1.0: cluster 0 =
+    row 0 alone, every bit invariant (mask all ones, bitval row 0); cluster 1
+    = rows 1, 2, which differ exactly in A's bits of column 0 (mask
+    0xFFFF0000 there, all ones elsewhere; bitval D in column 1)."""
+    masks = np.full((2, 33), 0xFFFFFFFF, dtype=np.uint32)
+    masks[1, 0] = 0xFFFF0000
+    vals = np.zeros((2, 33), dtype=np.uint32)
+    vals[0, 0] = 0x0000FFFF
+    vals[1, 1] = 0xFF000000
+    return masks, vals, np.array([0, 1, 1], dtype=np.int32)
+
+
+def test_kmeans_or_chain_known_answer(oracle):
+    """V > 32: the never-cleared accumulator of construct_bitmasks moves row 1
+    to cluster 1 (an accumulator reset per column, or another stride, leaves it
+    in cluster 0)."""
+    assert list(oracle.random_shuffle(3, seed=1)) == [0, 2, 1]
+    m, b, cl = oracle.kmeans(or_chain_data(), 2, seed=1)
+    em, eb, ecl = or_chain_expected()
+    assert np.array_equal(cl, ecl)
+    assert np.array_equal(m.view(np.uint32), em) and np.array_equal(b.view(np.uint32), eb)
