@@ -128,6 +128,18 @@ int ibp_condensed_offsets(int elem_bytes, const void *data, int64_t num_vecs, in
                           const void *mask, const void *bitval, const void *index, int index_bytes,
                           int64_t n, int64_t *offsets, const ibp_layout *layout, void *stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc):
+ * ibp_condensed_offsets with no allocation of any kind.  The scan's tile sums
+ * live in a fixed device array owned by the library (512 KiB per device, held
+ * until the process ends), so n is limited to 268,435,456 rows (larger n is
+ * IBP_ERR_INVALID) and offsets must be device memory.  Every call on a device
+ * shares that array: calls that overlap on one device (e.g. on two streams)
+ * race and must be ordered by the caller.  Asynchronous. */
+int ibp_condensed_offsets_noalloc(int elem_bytes, const void *data, int64_t num_vecs, int64_t vec_size,
+                                  const void *mask, const void *bitval, const void *index, int index_bytes,
+                                  int64_t n, int64_t *offsets, const ibp_layout *layout, void *stream);
+
 /* Replaces compress_inplace (src/compress.cu:176-236) -> ibp::compress_inplace
  * (include/compress/ibp_compress_host.cuh:8-27).  Slot i of `output` receives
  * row index[i] (or row i) of `input`, packed when that makes it smaller.

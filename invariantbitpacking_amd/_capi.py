@@ -56,6 +56,8 @@ SIGNATURES = {
     "ibp_get_compress_size_ex": (_I32, [_I32, _P, _I64, _I64, _P, _P, _P, _I32, _I64, _P, _P,
                                         ctypes.POINTER(Layout), _P]),
     "ibp_condensed_offsets": (_I32, [_I32, _P, _I64, _I64, _P, _P, _P, _I32, _I64, _P, ctypes.POINTER(Layout), _P]),
+    "ibp_condensed_offsets_noalloc": (_I32, [_I32, _P, _I64, _I64, _P, _P, _P, _I32, _I64, _P,
+                                             ctypes.POINTER(Layout), _P]),
     "ibp_compress_ex": (_I32, [_I32, _P, _P, _I64, _I64, _P, _P, _P, _I32, _I64, _P, ctypes.POINTER(Layout), _P]),
     "ibp_decompress_fetch_ex": (_I32, [_I32, _P, _P, _I64, _I64, _P, _P, _P, _P, _I32, _I64, _I32, _I32, _I32,
                                        _I32, _I32, ctypes.POINTER(Layout), _P]),

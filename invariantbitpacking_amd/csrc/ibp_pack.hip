@@ -15,6 +15,7 @@
 // and then written out, so in-place packing never overwrites input the wave
 // still has to read.  Words of 1 or 2 bytes are OR-ed through the 32-bit word
 // that contains them (LDS has no sub-dword atomics).
+#include <mutex>
 #include <type_traits>
 
 #include "ibp_bits.h"
@@ -467,20 +468,58 @@ int size_any(int elem_bytes, int index_bytes, const void *data, int64_t num_vecs
     }
 }
 
-// In-place inclusive scan of v[n] (device memory) on `stream`.
-int scan_inclusive(int64_t *v, int64_t n, hipStream_t stream)
+// In-place inclusive scan of v[n] (device memory) on `stream`.  The tile
+// sums (one int64 per kScanTile values) go to `tile_sums` when given (device
+// memory of at least that many values), else to a stream-ordered allocation.
+// Every tile sum is written before it is read: nothing needs clearing.
+int scan_inclusive(int64_t *v, int64_t n, hipStream_t stream, int64_t *tile_sums = nullptr)
 {
     if (n <= 0) return IBP_OK;
     const int64_t ntiles = (n + kScanTile - 1) / kScanTile;
-    int64_t *sums = nullptr;
-    IBP_HIP(hipMallocAsync((void **)&sums, (size_t)ntiles * sizeof(int64_t), stream));
+    int64_t *sums = tile_sums;
+    if (!sums) IBP_HIP(hipMallocAsync((void **)&sums, (size_t)ntiles * sizeof(int64_t), stream));
     hipLaunchKernelGGL(tile_sum_kernel, dim3((unsigned)ntiles), dim3(256), 0, stream, (const int64_t *)v, n, sums);
     hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(1024), 0, stream, sums, ntiles);
     hipLaunchKernelGGL(tile_scan_kernel, dim3((unsigned)ntiles), dim3(256), 0, stream, v, n, (const int64_t *)sums);
     hipError_t e = hipGetLastError();
-    IBP_HIP(hipFreeAsync(sums, stream));
+    if (!tile_sums) IBP_HIP(hipFreeAsync(sums, stream));
     if (e != hipSuccess) return set_error(IBP_ERR_HIP, "scan kernel launch failed: %s", hipGetErrorString(e));
     return IBP_OK;
+}
+
+// Scratch of ibp_condensed_offsets_noalloc: the tile sums of its offsets
+// scan.  kNoallocScanTiles values of 8 bytes (512 KiB), so up to
+// kNoallocScanTiles * kScanTile = 268,435,456 rows per call.  Each device holds
+// its own copy for the life of the process; calls on one device share it.
+constexpr int64_t kNoallocScanTiles = 65536;
+static __device__ __align__(256) int64_t scratch[kNoallocScanTiles];
+
+std::mutex g_scratch_mu;
+int64_t *g_scratch_ptr[64] = {nullptr};
+
+// The device address of `scratch` on the current device, looked up once per
+// device with that device current.  (In host code `scratch` names a host-side
+// stand-in, never the device memory.)  Sets the error and returns nullptr on
+// failure.
+int64_t *noalloc_scratch()
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess || dev < 0 || dev >= 64) {
+        set_error(IBP_ERR_HIP, "hipGetDevice: %s", hipGetErrorString(e));
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            set_error(IBP_ERR_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+            return nullptr;
+        }
+        g_scratch_ptr[dev] = (int64_t *)ptr;
+    }
+    return g_scratch_ptr[dev];
 }
 
 template <typename T, typename IdxT, bool CONDENSED>
@@ -619,6 +658,37 @@ extern "C" int ibp_condensed_offsets(int elem_bytes, const void *data, int64_t n
         if (st == IBP_OK) IBP_HIP(hipMemcpyAsync(offsets, dev, (size_t)n * sizeof(int64_t), hipMemcpyDeviceToHost, s));
         IBP_HIP(hipFreeAsync(dev, s));
     }
+    return st;
+}
+
+extern "C" int ibp_condensed_offsets_noalloc(int elem_bytes, const void *data, int64_t num_vecs,
+                                             int64_t vec_size, const void *mask, const void *bitval,
+                                             const void *index, int index_bytes, int64_t n, int64_t *offsets,
+                                             const ibp_layout *layout, void *stream)
+{
+    IBP_CHECK_WIDTH(elem_bytes);
+    IBP_CHECK(vec_size > 0 && num_vecs >= 0 && n >= 0, "invalid shape");
+    IBP_CHECK(index == nullptr || index_bytes == 4 || index_bytes == 8, "index must be int32 or int64");
+    IBP_CHECK(index != nullptr || n <= num_vecs, "without an index n must not exceed num_vecs");
+    IBP_CHECK(layout == nullptr || layout->offsets == nullptr, "layout->offsets is the output here: pass it as offsets");
+    IBP_CHECK(n <= kNoallocScanTiles * kScanTile,
+              "n = %lld rows exceeds the allocation-free limit of %lld rows (use ibp_condensed_offsets)",
+              (long long)n, (long long)(kNoallocScanTiles * kScanTile));
+    const int32_t *clusters = layout ? layout->clusters : nullptr;
+    const int64_t k = layout && clusters ? layout->num_clusters : 1;
+    IBP_CHECK(k >= 1, "num_clusters must be >= 1");
+    // Resolved before any early return, so the first call on a device sets up
+    // its scratch address whatever its size.
+    int64_t *sums = noalloc_scratch();
+    if (!sums) return IBP_ERR_HIP;
+    if (n == 0) return IBP_OK;
+    IBP_CHECK(data && mask && bitval && offsets, "null pointer argument");
+    IBP_CHECK(!pointer_is_host(offsets),
+              "offsets must be device memory here (host offsets need a staging buffer: use ibp_condensed_offsets)");
+    hipStream_t s = as_stream(stream);
+    int st = size_any(elem_bytes, index_bytes, data, num_vecs, vec_size, mask, bitval, clusters, k, index, n,
+                      offsets, nullptr, s);
+    if (st == IBP_OK) st = scan_inclusive(offsets, n, s, sums);
     return st;
 }
 
