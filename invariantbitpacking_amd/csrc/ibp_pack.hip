@@ -356,6 +356,147 @@ __global__ __launch_bounds__(1024) void pack_kernel(PackParams p)
     if (p.err && bad && lane == 0) atomicAdd(p.err, bad);
 }
 
+// Pack of rows of <= 64 elements of 4 or 8 bytes with one mask (the GNN and
+// DLRM shapes).  A wave takes whole 32-row groups (one bitmask word each) in
+// batches of PB rows: while batch b is packed, the rows of batch b + 1 --
+// and their indices and stream offsets -- are already being loaded, so a
+// wave keeps up to 2 * PB rows in flight instead of one.  Each row is packed
+// as in pack_kernel: lane j holds element j; one DPP scan of the payload
+// lengths gives every lane its bit position and the row's size; the slot is
+// OR-ed together in the wave's LDS buffer and read back by exchanging every
+// word with zero, which leaves the buffer clean for the next row.
+constexpr int kPackBatch = 8;
+
+template <typename T, typename IdxT, bool CONDENSED>
+__global__ __launch_bounds__(256) void pack_small_kernel(PackParams p)
+{
+    static_assert(sizeof(T) >= 4, "4- and 8-byte words");
+    constexpr int B = Bits<T>::B;
+    constexpr int W = (int)sizeof(T);
+    constexpr int PB = kPackBatch;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int lane = lane_id();
+    const int64_t wave = global_wave_id();
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
+    const int64_t V = p.V;
+    const int64_t buf_bytes = slot_buf_bytes(V, W);
+    T *buf = (T *)(lds_raw + (int64_t)wave_in_block_id() * buf_bytes);
+    const T *in = (const T *)p.in;
+    const int64_t Hw = header_bytes(V, W) / W;
+    const int64_t vec_bytes = V * W;
+    const bool in_place = !CONDENSED && p.in == (const void *)p.out && p.index == nullptr;
+    const int64_t ngroups = (p.n + 31) / 32;
+    unsigned long long bad = 0;
+    const SmallCols<T> col = load_small_cols<T>(p.mask, p.bitval, (int)V);
+    const int len_raw = col.valid ? B : 0;
+    const int64_t colj = col.valid ? lane : 0;
+    for (int64_t k = lane; k < buf_bytes / 4; k += kWave) ((uint32_t *)buf)[k] = 0u;
+    slot_fence<true>();
+    // This wave's batches: group g = wave + (b / 4) * nwaves, rows
+    // g * 32 + (b % 4) * PB + q.
+    const int64_t my_groups = wave < ngroups ? (ngroups - 1 - wave) / nwaves + 1 : 0;
+    const int64_t nbatch = my_groups * (32 / PB);
+    // Per batch, lane q < PB holds row q's source row (-1: invalid or past n)
+    // and, for a condensed output, its byte range [lo, hi).
+    auto batch_meta = [&](int64_t b, int64_t &src_row, int64_t &lo, int64_t &hi) {
+        const int64_t i = (wave + (b / (32 / PB)) * nwaves) * 32 + (b % (32 / PB)) * PB + (lane % PB);
+        src_row = -1;
+        lo = hi = 0;
+        if (i < p.n) {
+            const int64_t r = row_of<IdxT>(p.index, i);
+            src_row = (r >= 0 && r < p.num_vecs) ? r : -1;
+            if constexpr (CONDENSED) {
+                lo = i > 0 ? p.offsets[i - 1] : 0;
+                hi = p.offsets[i];
+            }
+        }
+    };
+    int64_t rc = -1, loc = 0, hic = 0, rn = -1, lon = 0, hin = 0;
+    T xc[PB], xn[PB];
+    if (nbatch > 0) {
+        batch_meta(0, rc, loc, hic);
+#pragma unroll
+        for (int q = 0; q < PB; ++q) {
+            const int64_t r = readlane(rc, q);
+            xc[q] = in[(r >= 0 ? r : 0) * V + colj];
+        }
+    }
+    uint32_t bits = 0;
+    for (int64_t b = 0; b < nbatch; ++b) {
+        // prefetch the next batch (the last batch re-reads itself)
+        batch_meta(b + 1 < nbatch ? b + 1 : b, rn, lon, hin);
+#pragma unroll
+        for (int q = 0; q < PB; ++q) {
+            const int64_t r = readlane(rn, q);
+            xn[q] = in[(r >= 0 ? r : 0) * V + colj];
+        }
+        const int64_t g = wave + (b / (32 / PB)) * nwaves;
+        const int64_t i0 = g * 32 + (b % (32 / PB)) * PB;
+#pragma unroll
+        for (int q = 0; q < PB; ++q) {
+            const int64_t i = i0 + q;
+            if (i >= p.n) break; // uniform
+            const int64_t r = readlane(rc, q);
+            if (r < 0) {
+                ++bad;
+                continue;
+            }
+            const T xs = col.valid ? xc[q] : T(0);
+            const bool fs = col.valid && (T)(xs & T(~col.free)) == col.bitval;
+            const int incl = wave_incl_scan(fs ? col.len_flagged : len_raw);
+            const int saved = (int)V * B - readlane(incl, kWave - 1);
+            const int64_t size = slot_size(V, W, saved);
+            T *dst;
+            if constexpr (CONDENSED) {
+                const int64_t lo = readlane(loc, q), hi = readlane(hic, q);
+                // The stored length must be this row's size: offsets from
+                // another dataset or mask never make a write leave the row.
+                if (lo < 0 || hi - lo != size || hi > p.stream_bytes || (lo % W) != 0) {
+                    ++bad;
+                    continue;
+                }
+                dst = (T *)((unsigned char *)p.out + lo);
+            } else {
+                dst = (T *)p.out + i * V;
+            }
+            if (size >= vec_bytes) {
+                if (!in_place && col.valid) dst[lane] = xs;
+                continue;
+            }
+            const uint64_t hb = ballot(fs);
+            if (lane < kWave / B && lane * B < V) {
+                const T hw = (T)(hb >> (lane * B));
+                if (hw) or_word<T>(buf, lane, hw);
+            }
+            const int L = fs ? col.len_flagged : len_raw;
+            const unsigned pbit = (unsigned)(incl - L);
+            const T v = fs ? small_extract<T>(col, xs) : xs;
+            const int64_t qw = Hw + pbit / B;
+            const int o = (int)(pbit % B);
+            if (L > 0) {
+                or_word<T>(buf, qw, shr<T>(v, o));
+                if (o + L > B) or_word<T>(buf, qw + 1, shl<T>(v, B - o));
+            }
+            slot_fence<true>();
+            const int64_t words = CONDENSED ? size / W : V; // the rest of the slot is zero
+            if (lane < words) dst[lane] = __hip_atomic_exchange(&buf[lane], T(0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            else if (lane < V) (void)__hip_atomic_exchange(&buf[lane], T(0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            slot_fence<true>(); // the exchanges land before the next row's ORs
+            bits |= 1u << (i - g * 32);
+        }
+        if ((b % (32 / PB)) == 32 / PB - 1) { // the group is complete
+            if (lane == 0 && p.bitmask) p.bitmask[g] = bits;
+            bits = 0;
+        }
+        rc = rn;
+        loc = lon;
+        hic = hin;
+#pragma unroll
+        for (int q = 0; q < PB; ++q) xc[q] = xn[q];
+    }
+    if (p.err && bad && lane == 0) atomicAdd(p.err, bad);
+}
+
 // ---- inclusive scan of int64 (the thrust::inclusive_scan of compress_condensed,
 // src/compress.cu:284-285): tile sums, an exclusive scan of the tile sums in
 // one block, then each tile scanned with its carry-in.
@@ -552,6 +693,20 @@ int pack_typed(PackParams p, hipStream_t stream)
         IBP_HIP(hipFreeAsync(p.gbuf, stream));
         if (e != hipSuccess) return set_error(IBP_ERR_HIP, "pack kernel launch failed: %s", hipGetErrorString(e));
         return IBP_OK;
+    }
+    if constexpr (sizeof(T) >= 4) {
+        if (p.V <= kWave && p.clusters == nullptr) {
+            // rows of <= 64 elements with one mask: batched loads, 8 blocks of
+            // four waves per CU
+            int64_t grid = (ngroups + 3) / 4;
+            if (grid > (int64_t)dev.num_cus * 8) grid = (int64_t)dev.num_cus * 8;
+            if (grid < 1) grid = 1;
+            p.gbuf = nullptr;
+            hipLaunchKernelGGL((pack_small_kernel<T, IdxT, CONDENSED>), dim3((unsigned)grid), dim3(4 * kWave),
+                               (size_t)(4 * buf_bytes), stream, p);
+            IBP_LAUNCH_CHECK();
+            return IBP_OK;
+        }
     }
     if (buf_bytes <= lds_budget) {
         int wpb = (int)(lds_budget / buf_bytes);

@@ -369,6 +369,134 @@ __global__ __launch_bounds__(256) void score_rows_kernel(const T *__restrict__ d
     if ((int)threadIdx.x < nt && threadIdx.x < NT && bsum[threadIdx.x]) atomicAdd(&scores[threadIdx.x], bsum[threadIdx.x]);
 }
 
+// Row-per-lane scoring from an LDS copy of a 64-row tile.  A tile's rows are
+// contiguous in memory, so a wave copies it into its LDS with C = row bytes
+// / 16 instructions of global_load_lds_dwordx4, each moving 1 KB of
+// consecutive bytes straight to LDS (no registers; the copy is tracked by
+// vmcnt).  Lane r then reads row r from LDS chunk by chunk, with the column
+// masks uniform scalar loads as in score_rows_kernel.  (Loading row r straight
+// into lane r makes every load instruction touch 64 rows 400 bytes apart, so
+// each 128-byte line is fetched again by later loads.)  Rows sit at their
+// natural stride; the kernel is used where that stride in dwords is not a
+// multiple of 16 modulo 32, so eight lanes reading 16 bytes each hit at most
+// two bank groups twice.  One wave per block, each with its own tile;
+// __syncthreads() orders the wave's copy and reads.
+constexpr int kTileMaxRowBytes = 512;
+__host__ __device__ constexpr bool tile_rows_ok(int64_t row_bytes)
+{
+    return row_bytes % 16 == 0 && row_bytes <= kTileMaxRowBytes && (row_bytes / 4) % 32 != 0 &&
+           (row_bytes / 4) % 32 != 16;
+}
+
+// Masks (NT) and bitvals (NB: one when the thresholds are nested) of column
+// j from the column-major tables, as uniform scalar loads.
+template <typename T, int NT, int NB>
+__device__ __forceinline__ void load_cols(const T *__restrict__ colm, const T *__restrict__ colb, int64_t j, T (&m)[NT],
+                                          T (&b)[NB])
+{
+#pragma unroll
+    for (int t = 0; t < NT; ++t) m[t] = colm[j * kMaxThresholds + t];
+#pragma unroll
+    for (int t = 0; t < NB; ++t) b[t] = colb[j * kMaxThresholds + t];
+}
+
+template <typename T, int NT, bool NESTED>
+__global__ __launch_bounds__(64) void score_tile_kernel(const T *__restrict__ data, int64_t num_vecs, int V,
+                                                        const T *__restrict__ colm, const T *__restrict__ colb,
+                                                        int nt, unsigned long long *scores)
+{
+    constexpr int EPC = 16 / (int)sizeof(T);
+    constexpr int NB = NESTED ? 1 : NT;
+    extern __shared__ __align__(16) unsigned char tile[];
+    const int lane = lane_id();
+    const int C = V * (int)sizeof(T) / 16; // chunks per row = copy instructions per tile
+    const int64_t ntiles = (num_vecs + kWave - 1) / kWave;
+    const int64_t nwaves = gridDim.x;
+    uint32_t pcsum[NT];
+    unsigned long long saved[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        pcsum[t] = 0;
+        saved[t] = 0;
+    }
+    for (int j = lane; j < V; j += kWave)
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+            if (t < nt) pcsum[t] += (uint32_t)popc((WideT<T>)colm[(int64_t)j * kMaxThresholds + t]);
+#pragma unroll
+    for (int t = 0; t < NT; ++t) pcsum[t] = (uint32_t)wave_sum((int)pcsum[t]);
+    const uint4 *mine = reinterpret_cast<const uint4 *>(tile) + lane * C;
+    for (int64_t tl = blockIdx.x; tl < ntiles; tl += nwaves) {
+        const uint4 *src = reinterpret_cast<const uint4 *>(data + tl * kWave * (int64_t)V);
+        const int64_t rows_left = num_vecs - tl * kWave;
+        // the last tile may be short: its missing rows re-read the last chunk
+        const int64_t last = (rows_left < kWave ? rows_left : kWave) * C - 1;
+        for (int k = 0; k < C; ++k) {
+            const int64_t g = (int64_t)k * kWave + lane;
+            __builtin_amdgcn_global_load_lds(static_cast<const void *>(src + (g < last ? g : last)),
+                                             (__attribute__((address_space(3))) void *)(tile + (size_t)k * (kWave * 16)),
+                                             16, 0, 0);
+        }
+        __builtin_amdgcn_s_waitcnt(0x0f70); // vmcnt(0): the copy has landed in LDS
+        __syncthreads();
+        const int64_t row = tl * kWave + lane;
+        uint32_t lost[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) lost[t] = 0;
+        // The column constants of element j + 1 are loaded while element j
+        // is scored: scalar loads complete in any order, so their only wait
+        // is lgkmcnt(0), which the empty asm statements place before the next
+        // loads are issued rather than after.
+        T mc[NT], bc[NB];
+        load_cols<T, NT, NB>(colm, colb, 0, mc, bc);
+        uint4 v = mine[0];
+        for (int c = 0; c < C; ++c) {
+#pragma unroll
+            for (int e = 0; e < EPC; ++e) {
+                const int64_t j = (int64_t)c * EPC + e;
+#pragma unroll
+                for (int t = 0; t < NT; ++t) asm volatile("" ::"s"(mc[t]));
+#pragma unroll
+                for (int t = 0; t < NB; ++t) asm volatile("" ::"s"(bc[t]));
+                asm volatile("" ::"v"(v.x), "v"(v.y), "v"(v.z), "v"(v.w) : "memory");
+                const T x = chunk_elem<T>(v, e);
+                T mn[NT], bn[NB];
+                load_cols<T, NT, NB>(colm, colb, j + 1 < V ? j + 1 : j, mn, bn);
+                if (e == EPC - 1 && c + 1 < C) v = mine[c + 1];
+                const T d0 = (T)(x ^ bc[0]);
+#pragma unroll
+                for (int t = 0; t < NT; ++t) {
+                    if (mc[t] == 0) continue; // uniform: no bits, nothing to lose
+                    const T d = NESTED ? d0 : (T)(x ^ bc[t]);
+                    lost[t] = lost_step(d, mc[t], (uint32_t)popc((WideT<T>)mc[t]), lost[t]);
+                }
+#pragma unroll
+                for (int t = 0; t < NT; ++t) mc[t] = mn[t];
+#pragma unroll
+                for (int t = 0; t < NB; ++t) bc[t] = bn[t];
+            }
+        }
+        if (row < num_vecs) {
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const uint32_t sv = pcsum[t] - lost[t];
+                if (t < nt && sv > (uint32_t)V) saved[t] += (unsigned long long)(sv - (uint32_t)V);
+            }
+        }
+        __syncthreads(); // this tile's reads are done before the next copy lands
+    }
+    // One atomic per threshold and wave: the lanes' savings are summed as
+    // 24-bit low parts and the rest (each sum of 64 fits 32 bits as long as a
+    // lane's saving is below 2^49).
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const uint32_t lo = (uint32_t)wave_sum((int)(uint32_t)(saved[t] & 0xffffffull));
+        const uint32_t hi = (uint32_t)wave_sum((int)(uint32_t)(saved[t] >> 24));
+        const unsigned long long tot = (unsigned long long)lo + ((unsigned long long)hi << 24);
+        if (lane == 0 && t < nt && tot) atomicAdd(&scores[t], tot);
+    }
+}
+
 // First strict maximum of the scores compared as double
 // (ibp_preproc_host.cuh:64,92-99); copies that threshold's mask/bitval out.
 template <typename T>
@@ -507,6 +635,33 @@ int preprocess_typed(const void *data, int64_t N, int64_t V, const Thresholds &t
         const bool rows_path = (V * (int64_t)sizeof(T)) % 16 == 0 && V <= (int64_t)1 << 20 &&
                                ((uintptr_t)data % 16) == 0 &&
                                (th.count > 1 || N >= (int64_t)dev.num_cus * 4 * kWave);
+        // Rows of up to 512 bytes whose stride spreads over the LDS banks: a
+        // wave per 64-row tile copied to LDS (25.6 KB at 400-byte rows: six
+        // such waves per CU).
+        const int64_t row_bytes = V * (int64_t)sizeof(T);
+        if (!scored && rows_path && tile_rows_ok(row_bytes)) {
+            const size_t lds = (size_t)kWave * (size_t)row_bytes;
+            const int64_t ntiles = (N + kWave - 1) / kWave;
+            int64_t per_cu = (int64_t)(160 * 1024) / (int64_t)lds;
+            if (per_cu > 8) per_cu = 8;
+            if (per_cu < 1) per_cu = 1;
+            int64_t grid = (int64_t)dev.num_cus * per_cu;
+            if (grid > ntiles) grid = ntiles;
+            const bool nested = thresholds_nested(th);
+            if (th.count <= 1)
+                hipLaunchKernelGGL((score_tile_kernel<T, 1, true>), dim3((unsigned)grid), dim3(kWave), lds, stream,
+                                   (const T *)data, N, (int)V, (const T *)colm, (const T *)colb, th.count, scores);
+            else if (nested)
+                hipLaunchKernelGGL((score_tile_kernel<T, kMaxThresholds, true>), dim3((unsigned)grid), dim3(kWave), lds,
+                                   stream, (const T *)data, N, (int)V, (const T *)colm, (const T *)colb, th.count,
+                                   scores);
+            else
+                hipLaunchKernelGGL((score_tile_kernel<T, kMaxThresholds, false>), dim3((unsigned)grid), dim3(kWave),
+                                   lds, stream, (const T *)data, N, (int)V, (const T *)colm, (const T *)colb, th.count,
+                                   scores);
+            IBP_LAUNCH_CHECK();
+            scored = true;
+        }
         if (!scored && rows_path) {
             int64_t grid = ((N + kWave - 1) / kWave + 3) / 4;
             if (grid > (int64_t)dev.num_cus * 8) grid = (int64_t)dev.num_cus * 8;
@@ -582,6 +737,32 @@ extern "C" int ibp_preprocess(int elem_bytes, const void *data, int64_t num_vecs
     case 2: st = preprocess_typed<uint16_t>(data, num_vecs, vec_size, th, mask, bitval, chosen_step, ws, s); break;
     case 4: st = preprocess_typed<uint32_t>(data, num_vecs, vec_size, th, mask, bitval, chosen_step, ws, s); break;
     default: st = preprocess_typed<uint64_t>(data, num_vecs, vec_size, th, mask, bitval, chosen_step, ws, s); break;
+    }
+    if (st == IBP_OK && debug_enabled()) {
+        // Debug mode reports what the reference's sweep prints
+        // (ibp_preproc_host.cuh:66,89-98): the bits each threshold saves, and
+        // each new best with its expected compressed length in elements.
+        // This synchronises the stream; without debug the call stays
+        // asynchronous.
+        unsigned long long sc[kMaxThresholds] = {0};
+        const PreWorkspace L = layout(elem_bytes, vec_size, th.count);
+        if (hipMemcpyAsync(sc, ws + L.scores_off, sizeof(sc), hipMemcpyDeviceToHost, s) == hipSuccess &&
+            hipStreamSynchronize(s) == hipSuccess) {
+            const double total = (double)num_vecs * (double)vec_size * 8.0 * elem_bytes;
+            dprintf("Num elems: %lld\n", (long long)num_vecs);
+            double best = 0;
+            for (int t = 0; t < th.count; ++t) {
+                dprintf("Threshold %.2f: Saved bits per element: %llu (Total %.0f, %.3f%%)\n", th.t[t], sc[t], total,
+                        total > 0 ? (double)sc[t] * 100.0 / total : 0.0);
+                if ((double)sc[t] > best) {
+                    best = (double)sc[t];
+                    const int len = total > 0 ? (int)((double)vec_size * (total - (double)sc[t]) / total) + 1 : 0;
+                    dprintf("Selected threshold %.2f; compress_len %d\n", th.t[t], len);
+                }
+            }
+        } else {
+            (void)hipGetLastError();
+        }
     }
     if (own) {
         hipError_t e = hipFreeAsync(ws, s);
