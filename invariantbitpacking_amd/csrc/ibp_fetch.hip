@@ -48,6 +48,17 @@ int fetch_launch(const FetchParams &p, int elem_bytes, int index_bytes, int impl
     if (host_resident && p.vec_size <= kWave) cap = dev.num_cus / 2 > 0 ? dev.num_cus / 2 : 1;
     if (nblks <= 0 && grid > cap) grid = cap;
     if (grid < 1) grid = 1;
+    if (debug_enabled()) {
+        // What the reference reports about its launch (ibp_decompress_host.cuh:
+        // 52-61): which codec runs, where the rows live, the launch shape.
+        const char *kind = (impl == 2 || impl == 3 || impl == 4) ? "transfer baseline"
+                           : p.vec_size <= kWave ? (layout == kLayoutSlots ? "wave per row group" : "wave per row group (layout)")
+                                                 : "wave per row";
+        dprintf("[ibp] decompress_fetch: %s, impl %d, %s rows, V=%lld, w=%d, n=%lld, grid %lld x %d, rows in flight %d, "
+                "first read %d words, comp_len %d\n",
+                kind, impl, host_resident ? "host (PCIe)" : "device", (long long)p.vec_size, elem_bytes,
+                (long long)p.n, (long long)grid, block, p.vec_size <= kWave ? rows : 1, p.first_words, comp_len);
+    }
     switch (elem_bytes) {
     case 1: return fetch_launch_width<uint8_t>(p, layout, index_bytes, impl, comp_len, rows, (int)grid, block, stream);
     case 2: return fetch_launch_width<uint16_t>(p, layout, index_bytes, impl, comp_len, rows, (int)grid, block, stream);

@@ -553,5 +553,30 @@ extern "C" int ibp_preprocess_kmeans(int elem_bytes, const void *data, int64_t n
                        : kmeans_typed<uint32_t>(data, num_vecs, vec_size, num_clusters, seed, masks, bitvals, clusters,
                                                 as_stream(stream));
     if (st == IBP_OK) dprintf("[ibp] kmeans: K=%d V=%lld N=%lld\n", num_clusters, (long long)vec_size, (long long)num_vecs);
+    if (st == IBP_OK && debug_enabled()) {
+        // The per-centroid report of the reference's debug build
+        // (ibp_preproc_host.cuh:221-232): invariant bits of each used
+        // centroid's mask and its member count.  The call has synchronised.
+        std::vector<int32_t> cl((size_t)num_vecs);
+        std::vector<unsigned char> mk((size_t)num_clusters * vec_size * elem_bytes);
+        if (hipMemcpy(cl.data(), clusters, cl.size() * sizeof(int32_t), hipMemcpyDefault) == hipSuccess &&
+            hipMemcpy(mk.data(), masks, mk.size(), hipMemcpyDefault) == hipSuccess) {
+            std::vector<int64_t> members((size_t)num_clusters, 0);
+            for (int32_t c : cl)
+                if (c >= 0 && c < num_clusters) ++members[(size_t)c];
+            for (int c = 0; c < num_clusters; ++c) {
+                if (!members[(size_t)c]) continue;
+                int64_t pc = 0;
+                for (int64_t j = 0; j < vec_size * elem_bytes; ++j)
+                    pc += __builtin_popcount(mk[(size_t)c * vec_size * elem_bytes + j]);
+                const double bits = (double)vec_size * 8.0 * elem_bytes;
+                dprintf("Centroid %d: Set bits %lld of %.0f (%f%%) for %lld nodes\n", c, (long long)pc, bits,
+                        (double)pc * 100.0 / bits, (long long)members[(size_t)c]);
+            }
+        } else {
+            (void)hipGetLastError();
+        }
+        dprintf("Finished clustering\n");
+    }
     return st;
 }
