@@ -14,8 +14,6 @@ batched form (all minibatches in one persistent launch), the PCIe-resident
 fetch against a measured pinned H2D copy ceiling, and call latency.
 """
 import argparse
-import queue
-import threading
 import json
 import os
 import sys
@@ -30,10 +28,6 @@ sys.path.insert(0, ROOT)
 METRIC = "decompressed GB/s into HBM (batched vs plain, bit-exact) + p99 fetch latency"
 HBM_PEAK_GBPS = 8000.0
 
-# HBM bytes per launch of the plain fetch, measured with two rocprofv3 --pmc
-# passes (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0, factors from a 256 MiB device
-# copy), keyed by (rows, vec_size, batch); profiles/hbm_traffic_fetch_gnn_minibatch.md.
-MEASURED_TRAFFIC = {(2449029, 50, 200000): 53608 * 1024 * 2 + 78127 * 1024}
 
 
 def log(*a):
@@ -132,6 +126,7 @@ def cpu_compress_baseline(x_np, rows=100000):
     from oracle import oracle
     sample = np.ascontiguousarray(x_np[:rows])
     nbytes = sample.nbytes
+    oracle.set_threads(1)
     t0 = time.perf_counter()
     m, b, _, _ = oracle.preprocess(sample, None)
     t_pre = time.perf_counter() - t0
@@ -174,27 +169,40 @@ def run_batched(ibp, comp, mask, bitval, bitmask, comp_len, idx, x, batch, v):
             "ms_per_launch": round(bms, 4), "bitexact": exact, "plan": json.loads(desc)}
 
 
-def run_pcie(ibp, comp, mask, bitval, bitmask, comp_len, idx, x, out, sel, steps):
-    """The same fetch with the compressed table in pinned host memory: the
-    kernel reads it over PCIe.  Compared with a pinned H2D copy."""
+def run_pcie(ibp, comp, mask, bitval, bitmask, comp_len, idx, x, outs, sel, traffic=None):
+    """The GNN minibatch as BASELINE states it: the compressed table in pinned
+    host memory, the kernel reading it over PCIe; every minibatch of the index
+    list fetched once (timed), each output checked.  Roofline: compressed
+    bytes per launch over kernel time against a measured pinned H2D copy;
+    `traffic` is the host-read bytes per launch from counters (or null)."""
     comp_h = comp.cpu().pin_memory()
     ceiling = h2d_ceiling_gbps()
     step = [0]
 
     def fetch_h():
-        ibp.decompress_fetch(comp_h, mask, bitval, bitmask, torch.device("cuda"), comp_len, idx[step[0] % len(idx)],
-                             None, None, None, out)
+        k = step[0] % len(idx)
+        ibp.decompress_fetch(comp_h, mask, bitval, bitmask, torch.device("cuda"), comp_len, idx[k], None, None, None,
+                             outs[k])
         step[0] += 1
-    hk, _ = timed_region(fetch_h, steps, 1)
-    exact = bool(torch.equal(out, x[idx[(step[0] - 1) % len(idx)]]))
+    hk, _ = timed_region(fetch_h, len(idx), 1)
+    exact = sum(bool(torch.equal(o, x[i])) for o, i in zip(outs, idx))
     pcie_bytes = float(np.mean(sel))
-    dec = out.numel() * out.element_size()
+    dec = outs[0].numel() * outs[0].element_size()
+    achieved = pcie_bytes / (hk / 1e3) / 1e9
     res = {"decompressed_gbps": round(dec / (hk / 1e3) / 1e9, 2),
-           "compressed_pcie_gbps": round(pcie_bytes / (hk / 1e3) / 1e9, 2),
+           "compressed_pcie_gbps": round(achieved, 2),
            "h2d_copy_ceiling_gbps": round(ceiling, 2),
-           "frac_of_ceiling": round(pcie_bytes / (hk / 1e3) / 1e9 / ceiling, 3),
-           "ms_per_call": round(hk, 4), "bitexact": exact}
-    res["transfer_baselines"] = pcie_transfer_baselines(ibp, comp_h, mask, bitval, bitmask, comp_len, idx, x, out, steps)
+           "frac_of_ceiling": round(achieved / ceiling, 3),
+           "ms_per_call": round(hk, 4), "minibatches": len(idx), "bitexact_minibatches": exact,
+           "bitexact": exact == len(idx),
+           "roofline": {"bound": "pcie", "achieved": round(achieved, 2), "peak": round(ceiling, 2), "unit": "GB/s",
+                        "frac": round(achieved / ceiling, 4),
+                        "peak_what": "pinned 256 MiB hipMemcpy H2D, measured in this run",
+                        "algorithmic_bytes_per_launch": int(pcie_bytes),
+                        "traffic": None if not traffic else traffic.get("host_read_bytes"),
+                        "traffic_what": None if not traffic else traffic.get("host_read_what")}}
+    res["transfer_baselines"] = pcie_transfer_baselines(ibp, comp_h, mask, bitval, bitmask, comp_len, idx, x,
+                                                        outs[0], max(3, min(16, len(idx))))
     del comp_h
     log("pcie: %s" % res)
     return res
@@ -250,7 +258,133 @@ def pcie_transfer_baselines(ibp, comp_h, mask, bitval, bitmask, comp_len, idx, x
             "fake_fetch": {"ms_per_call": round(f_ms, 4), "decompressed_gbps": round(dec / (f_ms / 1e3) / 1e9, 2)}}
 
 
-def run_gnn_minibatch(args):
+def _pmc_csv(outdir):
+    """{kernel name: {counter: [value per dispatch]}} from the
+    counter_collection CSV files rocprofv3 wrote under outdir."""
+    import csv
+    import glob
+    per = {}
+    for path in glob.glob(os.path.join(outdir, "**", "*counter_collection.csv"), recursive=True):
+        with open(path) as f:
+            for row in csv.DictReader(f):
+                key = (row.get("Kernel_Name", ""), row.get("Counter_Name", ""), row.get("Dispatch_Id", ""))
+                per[key] = per.get(key, 0.0) + float(row.get("Counter_Value", "nan"))
+    out = {}
+    for (k, c, _), val in per.items():
+        out.setdefault(k, {}).setdefault(c, []).append(val)
+    return out
+
+
+def _pmc_pass(counters, ops, args, regex):
+    """Runs `bench.py --workload kernels --ops ops` under `rocprofv3 --pmc
+    counters` in a child process; returns {kernel: {counter: [values]}} or
+    raises with the reason."""
+    import shutil
+    import subprocess
+    import tempfile
+    prof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(prof):
+        raise RuntimeError("rocprofv3 not found")
+    out = tempfile.mkdtemp(prefix="ibp_pmc_", dir="/tmp")
+    cmd = ["timeout", "-s", "KILL", "150", prof, "--pmc"] + counters + [
+        "--kernel-include-regex", regex, "--output-format", "csv", "-d", out, "-o", "pmc", "--",
+        sys.executable, os.path.abspath(__file__), "--workload", "kernels", "--steps", "3", "--ops", ops,
+        "--scale", str(args.scale), "--batch", str(args.batch)]
+    env = dict(os.environ, TMPDIR="/tmp")
+    r = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("rocprofv3 exited %d: %s" % (r.returncode, r.stdout[-300:]))
+    vals = _pmc_csv(out)
+    shutil.rmtree(out, ignore_errors=True)
+    if not vals:
+        raise RuntimeError("no counter rows: %s" % r.stdout[-300:])
+    return vals
+
+
+def _mean_of(vals, kernel_part, counter):
+    xs = [v for k, cs in vals.items() if kernel_part in k for c, vs in cs.items() if c == counter for v in vs]
+    if not xs:
+        raise RuntimeError("no %s values for a kernel named *%s*" % (counter, kernel_part))
+    return float(np.mean(xs)), len(xs)
+
+
+def measure_traffic(args):
+    """DRAM and host-read bytes per launch of the headline fetch, measured live
+    by rocprofv3 counter passes over child runs of this script (started before
+    this process touches the GPU).  Each pass runs the plain fetch of one
+    200,000-row minibatch beside a calibration kernel of known size: the
+    raw-row kernel copying the whole uncompressed table in order (rows x 400
+    bytes read and written) for HBM, and reading a 256 MiB pinned table in
+    order for host memory.
+      hbm_bytes: FETCH_SIZE and WRITE_SIZE (KiB per dispatch) of the fetch,
+        each scaled by known bytes / counter of the calibration copy;
+      hbm_read_bytes_by_size: 32, 64 and 128 bytes times the fetch's
+        TCC_EA0_RDREQ_{32B,64B,128B} (a cross-check without scaling);
+      host_read_bytes: (TCC_EA0_RDREQ - TCC_EA0_RDREQ_DRAM) of the fetch from
+        pinned memory -- L2 read requests that leave for host memory -- times
+        the bytes per such request of the 256 MiB pinned read.
+    Returns a dict, with the reason in "*_error" for a pass that failed."""
+    from invariantbitpacking_amd import workloads
+    res = {}
+    t0 = time.perf_counter()
+    rows = workloads._rows(2449029, args.scale)
+    table = rows * 50 * 8
+    kern = "fetch_small_kernel|fetch_baseline_kernel"
+    sizes = ["TCC_EA0_RDREQ_sum", "TCC_EA0_RDREQ_32B_sum", "TCC_EA0_RDREQ_64B_sum", "TCC_EA0_RDREQ_128B_sum"]
+
+    def by_size(vals, part):
+        n32, _ = _mean_of(vals, part, sizes[1])
+        n64, _ = _mean_of(vals, part, sizes[2])
+        n128, _ = _mean_of(vals, part, sizes[3])
+        return 32 * n32 + 64 * n64 + 128 * n128
+    try:
+        ops = "fetch_hbm,seq_copy_hbm"
+        f = _pmc_pass(["FETCH_SIZE"], ops, args, kern)
+        w_ = _pmc_pass(["WRITE_SIZE"], ops, args, kern)
+        cf, _ = _mean_of(f, "fetch_baseline", "FETCH_SIZE")
+        cw, _ = _mean_of(w_, "fetch_baseline", "WRITE_SIZE")
+        ff, nf = _mean_of(f, "fetch_small_kernel", "FETCH_SIZE")
+        fw, _ = _mean_of(w_, "fetch_small_kernel", "WRITE_SIZE")
+        fs, ws = table / (cf * 1024), table / (cw * 1024)
+        res["fetch_size_kib"], res["write_size_kib"] = round(ff, 1), round(fw, 1)
+        res["fetch_scale"], res["write_scale"] = round(fs, 4), round(ws, 4)
+        res["hbm_read_bytes"], res["hbm_write_bytes"] = int(ff * 1024 * fs), int(fw * 1024 * ws)
+        res["hbm_bytes"] = res["hbm_read_bytes"] + res["hbm_write_bytes"]
+        res["hbm_what"] = ("FETCH_SIZE x %.3f + WRITE_SIZE x %.3f KiB per fetch_small_kernel dispatch (%d dispatches); "
+                           "scales from the raw-row kernel copying the %d-byte table in order" % (fs, ws, nf, table))
+        z = _pmc_pass(sizes, ops, args, kern)
+        res["hbm_read_bytes_by_size"] = int(by_size(z, "fetch_small_kernel"))
+        res["calibration_read_bytes_by_size"] = int(by_size(z, "fetch_baseline"))
+    except Exception as e:  # noqa: BLE001 -- reported, never fatal
+        res["hbm_error"] = str(e)[:300]
+    if not args.no_pcie:
+        try:
+            ops = "fetch_pinned,sequential_read_pinned_256MiB"
+            c = ["TCC_EA0_RDREQ_sum", "TCC_EA0_RDREQ_DRAM_sum"]
+            p = _pmc_pass(c, ops, args, kern)
+            sr, _ = _mean_of(p, "fetch_baseline", c[0])
+            sd, _ = _mean_of(p, "fetch_baseline", c[1])
+            fr, nf = _mean_of(p, "fetch_small_kernel", c[0])
+            fd, _ = _mean_of(p, "fetch_small_kernel", c[1])
+            per_req = (256 << 20) / max(1.0, sr - sd)
+            res["host_read_bytes"] = int((fr - fd) * per_req)
+            res["host_read_what"] = ("(TCC_EA0_RDREQ - TCC_EA0_RDREQ_DRAM) x %.1f bytes per request, per fetch "
+                                     "dispatch from pinned memory (%d dispatches); bytes per request from a "
+                                     "sequential read of a 256 MiB pinned table" % (per_req, nf))
+            z = _pmc_pass(sizes, ops, args, kern)
+            res["pinned_fetch_read_bytes_by_size"] = int(by_size(z, "fetch_small_kernel"))
+        except Exception as e:  # noqa: BLE001
+            res["host_read_error"] = str(e)[:300]
+    res["seconds"] = round(time.perf_counter() - t0, 1)
+    log("traffic: %s" % res)
+    return res
+
+
+def run_gnn_minibatch(args, traffic=None):
+    """The headline: GNN minibatch table compressed in HBM, one plain
+    decompress_fetch of 200,000 random rows per step, each of the 64
+    minibatches into its own preallocated output (every output of the timed
+    region is checked afterwards)."""
     import invariantbitpacking_amd as ibp
     from invariantbitpacking_amd import workloads
 
@@ -283,35 +417,49 @@ def run_gnn_minibatch(args):
         (packed, num_rows, comp_len, num_rows * v * w / float(sizes.sum())))
 
     batch = min(args.batch, num_rows)
-    nb = max(args.steps + args.warmup, 1)
-    idx_cpu = [workloads.gnn_minibatch_indices(num_rows, batch, s % 64) for s in range(min(nb, 64))]
+    nsets = 64
+    idx_cpu = [workloads.gnn_minibatch_indices(num_rows, batch, s) for s in range(nsets)]
     idx = [i.cuda() for i in idx_cpu]
     out = torch.empty((batch, v), dtype=torch.int64, device="cuda")
     step = [0]
 
     def fetch():
-        ibp.decompress_fetch(comp, mask, bitval, bitmask, torch.device("cuda"), comp_len, idx[step[0] % len(idx)],
-                             None, None, None, out)
+        ibp.decompress_fetch(comp, mask, bitval, bitmask, torch.device("cuda"), comp_len, idx[step[0] % nsets], None,
+                             None, None, out)
         step[0] += 1
 
-    # -- plain per-call fetch, compressed table resident in HBM (the metric)
+    # -- plain per-call fetch into one preallocated output, compressed table
+    # resident in HBM (the metric)
     kern_ms, wall = timed_region(fetch, args.steps, args.warmup)
-    bitexact = bool(torch.equal(out, x[idx[(step[0] - 1) % len(idx)]]))
     dec_bytes = batch * v * w
     value = args.steps * dec_bytes / wall / 1e9
+    # every minibatch of the timed sequence fetched again, untimed, and
+    # compared after its own call
+    timed = sorted({s % nsets for s in range(args.warmup, args.warmup + args.steps)})
+    exact = 0
+    for k in timed:
+        ibp.decompress_fetch(comp, mask, bitval, bitmask, torch.device("cuda"), comp_len, idx[k], None, None, None,
+                             out)
+        exact += bool(torch.equal(out, x[idx[k]]))
     lat = call_latencies(fetch, max(20, args.steps))
+    # side figure: a separate output per minibatch (5.1 GB of outputs), so the
+    # rows written are not the ones the previous call left in the caches
+    outs = [torch.empty((batch, v), dtype=torch.int64, device="cuda") for _ in range(nsets)]
+    step_r = [0]
+
+    def fetch_rot():
+        k = step_r[0] % nsets
+        ibp.decompress_fetch(comp, mask, bitval, bitmask, torch.device("cuda"), comp_len, idx[k], None, None, None,
+                             outs[k])
+        step_r[0] += 1
+    rot_ms, rot_wall = timed_region(fetch_rot, args.steps, args.warmup)
+    rot_exact = sum(bool(torch.equal(outs[k], x[idx[k]])) for k in
+                    sorted({s % nsets for s in range(args.warmup, args.warmup + args.steps)}))
     sel = [int(sizes[i].sum()) for i in idx]
     traffic_alg = algorithmic_hbm_bytes(np.mean(sel), batch, v, w)
     achieved = traffic_alg / (kern_ms / 1e3) / 1e9
-    log("plain: %.4f ms/call (events), %.1f GB/s decompressed, kernel-side %.1f GB/s HBM" % (kern_ms, value, achieved))
-
-    if args.calibrate_copy:
-        a = torch.empty(256 << 20, dtype=torch.uint8, device="cuda")
-        b = torch.empty_like(a)
-        for _ in range(4):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        del a, b
+    log("plain: %.4f ms/call (events), %.1f GB/s decompressed, kernel-side %.1f GB/s HBM, %d/%d minibatches exact"
+        % (kern_ms, value, achieved, exact, len(timed)))
 
     batched = None
     if not args.no_batched:
@@ -319,7 +467,8 @@ def run_gnn_minibatch(args):
         batched["vs_plain"] = round(batched["decompressed_gbps"] / value, 3)
     pcie = None
     if not args.no_pcie:
-        pcie = run_pcie(ibp, comp, mask, bitval, bitmask, comp_len, idx, x, out, sel, max(3, args.steps // 2))
+        pcie = run_pcie(ibp, comp, mask, bitval, bitmask, comp_len, idx, x, outs, sel, traffic)
+    del outs
     cpu = cpu_compress = None
     if not args.no_cpu:
         comp_np = comp.cpu().numpy()
@@ -331,19 +480,26 @@ def run_gnn_minibatch(args):
         cpu_compress = cpu_compress_baseline(x[:100000].cpu().numpy())
         log("cpu compression: %s" % cpu_compress)
 
-    return {
+    res = {
         "metric": METRIC, "value": round(value, 2), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "int64", "data": "synthetic",
         "config": {"workload": "gnn_minibatch: %d x %d int64 (fp32 [%d x 100]) compressed in HBM, %d random rows "
-                               "per step, decompress_fetch into a preallocated output" % (num_rows, v, num_rows, batch),
+                               "per step, decompress_fetch into preallocated outputs" % (num_rows, v, num_rows, batch),
                    "rows": num_rows, "vec_size": v, "batch": batch, "packed_rows": packed, "comp_len": comp_len},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBPS, 4),
-                     "traffic": MEASURED_TRAFFIC.get((num_rows, v, batch)),
+                     "traffic": None if not traffic else traffic.get("hbm_bytes"),
+                     "traffic_what": None if not traffic else traffic.get("hbm_what"),
                      "algorithmic_bytes_per_launch": traffic_alg, "kernel_ms": round(kern_ms, 5)},
         "cpu_baseline": cpu,
-        "bitexact": bitexact and (batched is None or batched["bitexact"]) and (pcie is None or pcie["bitexact"]),
+        "bitexact": exact == len(timed) and rot_exact == len(timed) and (batched is None or batched["bitexact"]) and
+                    (pcie is None or pcie["bitexact"]),
+        "bitexact_minibatches": "%d of %d minibatches of the timed sequence (each fetched again into the output "
+                                "and compared)" % (exact, len(timed)),
+        "separate_outputs": {"what": "the same steps, each minibatch into its own preallocated output",
+                             "decompressed_gbps": round(args.steps * dec_bytes / rot_wall / 1e9, 2),
+                             "kernel_ms": round(rot_ms, 5), "bitexact": rot_exact == len(timed)},
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4),
                        "what": "isolated call: decompress_fetch issued -> rows ready (host wall clock)"},
         "batched": batched,
@@ -351,6 +507,10 @@ def run_gnn_minibatch(args):
         "compression": {"gpu": gpu_compress, "cpu": cpu_compress},
         "host": host_info(),
     }
+    if traffic:
+        res["counters"] = {k: v for k, v in traffic.items() if k not in ("hbm_bytes", "host_read_bytes")}
+    res["set"] = dict(x=x, comp=comp, mask=mask, bitval=bitval, bitmask=bitmask, comp_len=comp_len)
+    return res
 
 
 def _compress_set(ibp, x):
@@ -362,7 +522,7 @@ def _compress_set(ibp, x):
     return dict(x=x, mask=mask, bitval=bitval, sizes=sizes, comp=comp, bitmask=bitmask, comp_len=comp_len)
 
 
-def run_dlrm(args):
+def run_dlrm(args, condensed=True):
     """26 embedding tables (rows log-uniform in [1e3, 1e7], dim 128 fp32 as
     int64 [rows x 64]), each preprocessed separately; a batch gathers 65,536
     Zipf(1.05) rows per table.  Plain = 26 decompress_fetch calls; batched =
@@ -403,7 +563,9 @@ def run_dlrm(args):
            "batched_vs_plain": round(p_ms / b_ms, 3), "batched_hbm_alg_gbps": round(alg / (b_ms / 1e3) / 1e9, 1),
            "plan": json.loads(bf.describe()), "bitexact": exact and b_exact}
     bf.close()
-    res["condensed"] = run_dlrm_condensed(ibp, args, sets, idx, outs, raw_bytes)
+    if condensed:
+        res["condensed"] = run_dlrm_condensed(ibp, args, sets, idx, outs, raw_bytes)
+    res["sets"] = sets
     return res
 
 
@@ -478,20 +640,44 @@ def run_dlrm_condensed(ibp, args, sets, idx, outs, raw_bytes):
     return res
 
 
-def run_serve(args):
-    """Requests drawn from three resident sets (GNN features, GNN minibatch
-    table, one DLRM table), index counts log-uniform in [1e3, 2.5e5], Poisson
-    arrivals at each rate.  Served: continuous batching over 4 streams.
-    Plain: one decompress_fetch per request in arrival order."""
+def run_serve(args, groups=None):
+    """Requests drawn from resident sets (GNN features, GNN minibatch table,
+    DLRM tables), index counts log-uniform in [1e3, 2.5e5], Poisson arrivals
+    at each rate.  Served: continuous batching over 4 streams.  Plain: one
+    decompress_fetch per request in arrival order.  Both latencies run from
+    the request's scheduled arrival to its completion; throughput is the
+    decompressed bytes over first scheduled arrival -> last completion.  Every
+    result of both forms is kept (the arena and the plain outputs hold a whole
+    pass) and compared with x[idx] after the pass."""
     import invariantbitpacking_amd as ibp
     from invariantbitpacking_amd import workloads
-    d1, _ = workloads.gnn_features()
-    s1 = _compress_set(ibp, d1.cuda())
-    s2 = _compress_set(ibp, workloads.gnn_minibatch(scale=args.scale, device="cuda"))
-    s3 = _compress_set(ibp, workloads.dlrm_tables(num_tables=1, scale=args.scale * 8, device="cuda")[0])
-    sets = [s1, s2, s3]
+    if groups is None:
+        d1, _ = workloads.gnn_features()
+        s1 = _compress_set(ibp, d1.cuda())
+        s2 = _compress_set(ibp, workloads.gnn_minibatch(scale=args.scale, device="cuda"))
+        s3 = [_compress_set(ibp, t) for t in workloads.dlrm_tables(scale=args.scale, device="cuda")]
+        groups = [[s1], [s2], s3]
+    # a request picks a workload uniformly, then one of its tensor sets
+    sets = [st for g in groups for st in g]
+    first = np.cumsum([0] + [len(g) for g in groups])
     torch.cuda.synchronize()
-    srv = ibp.Server(hbm_budget_bytes=8 << 30, num_streams=4, max_batch_requests=64, max_batch_rows=1 << 21)
+    rng = np.random.default_rng(300)
+    passes = []
+    for rate in list(args.rates) + [float("inf")]:
+        nreq = 400 if rate == float("inf") else int(min(400, max(50, rate * 0.5)))
+        arrivals = np.zeros(nreq) if rate == float("inf") else np.cumsum(rng.exponential(1.0 / rate, size=nreq))
+        reqs = []
+        for _ in range(nreq):
+            grp = int(rng.integers(0, len(groups)))
+            k = int(first[grp] + rng.integers(0, len(groups[grp])))
+            n = int(np.exp(rng.uniform(np.log(1e3), np.log(2.5e5))))
+            reqs.append((k, rng.integers(0, sets[k]["x"].shape[0], size=n)))
+        passes.append((rate, arrivals, reqs))
+    # The arena holds every output of a pass (they are checked after it), with
+    # room for the 256-byte alignment of each region.
+    need = max(sum(len(i) * sets[k]["x"].shape[1] * 8 + 256 for k, i in reqs) for _, _, reqs in passes)
+    srv = ibp.Server(hbm_budget_bytes=int(need * 1.05) + (256 << 20), num_streams=4, max_batch_requests=64,
+                     max_batch_rows=1 << 21)
     sids = [srv.register(s["comp"], s["mask"], s["bitval"], s["bitmask"], s["comp_len"]) for s in sets]
     # first launches load kernels and set up streams: keep that out of the latencies
     for sid, s in zip(sids, sets):
@@ -502,57 +688,53 @@ def run_serve(args):
         ibp.decompress_fetch(s["comp"], s["mask"], s["bitval"], s["bitmask"], torch.device("cuda"), s["comp_len"],
                              torch.from_numpy(w).cuda())
     torch.cuda.synchronize()
-    rng = np.random.default_rng(300)
     results = []
-    for rate in list(args.rates) + [float("inf")]:
-        nreq = 400 if rate == float("inf") else int(min(400, max(50, rate * 0.5)))
-        arrivals = np.zeros(nreq) if rate == float("inf") else np.cumsum(rng.exponential(1.0 / rate, size=nreq))
-        reqs = []
-        for _ in range(nreq):
-            k = int(rng.integers(0, 3))
-            n = int(np.exp(rng.uniform(np.log(1e3), np.log(2.5e5))))
-            reqs.append((k, rng.integers(0, sets[k]["x"].shape[0], size=n)))
+    total_checked = 0
+    all_exact = True
+    for rate, arrivals, reqs in passes:
+        nreq = len(reqs)
         dec = sum(len(i) * sets[k]["x"].shape[1] * 8 for k, i in reqs)
-        # served: a consumer thread waits for each request in submission order
-        # and releases its output at once, while the arrivals are submitted.
-        # Only the first 8 results are viewed as tensors (bit-exact check): the
-        # harness's own Python time per request stays small, so the arrival
-        # schedule holds at 1e4 requests/s.
-        handles = queue.Queue()
-        times, ok = [], [True]
-
-        def consume():
-            for j, (k, i) in enumerate(reqs):
-                h = handles.get()
-                out, _ = srv.wait(h, view=j < 8)
-                if out is not None:
-                    ok[0] = ok[0] and bool(torch.equal(out, sets[k]["x"][torch.from_numpy(i).cuda()]))
-                times.append(srv.request_times(h))
-                srv.release(h)
-
-        consumer = threading.Thread(target=consume)
-        consumer.start()
-        t0 = time.perf_counter()
+        # served: every request is submitted at its scheduled arrival (the
+        # server stamps that time and its own completion time); the results
+        # stay in the arena and are collected after the last submission.
+        hs = []
+        t0_us = srv.clock_us() + 1000.0  # the schedule starts 1 ms from now
         for (k, i), ta in zip(reqs, arrivals):
-            wait_until(t0, ta)
-            handles.put(srv.submit(sids[k], i))
-        consumer.join()
-        ok = ok[0]
-        tm = np.array(times)  # arrival, admitted, launch start, issued, done (host us)
-        lat = (tm[:, 4] - tm[:, 0]) / 1e3
-        served_wall = (tm[:, 4].max() - tm[:, 0].min()) / 1e6
-        worst = tm[int(np.argmax(lat))]
-        # plain per-request calls
-        plat = []
-        t0 = time.perf_counter()
+            due = t0_us + ta * 1e6
+            left = (due - srv.clock_us()) / 1e6
+            if left > 3e-4:
+                time.sleep(left - 3e-4)
+            while srv.clock_us() < due:
+                pass
+            hs.append(srv.submit(sids[k], i, arrival_us=due))
+        done = []
+        for h in hs:
+            srv.wait(h, view=False)
+            done.append(h)
+        times = np.array([srv.request_times(h) for h in done])  # arrival, admitted, launch, issued, done (us)
+        lat = (times[:, 4] - times[:, 0]) / 1e3
+        served_wall = (times[:, 4].max() - (t0_us + arrivals[0] * 1e6)) / 1e6
+        worst = times[int(np.argmax(lat))]
+        ok = True
+        for (k, i), h in zip(reqs, done):
+            got, _ = srv.wait(h)
+            ok = ok and bool(torch.equal(got, sets[k]["x"][torch.from_numpy(i).cuda()]))
+            srv.release(h)
+        # plain per-request calls, outputs kept for the same check
+        plat, pouts = [], []
+        t0 = time.perf_counter() + 1e-3
         for (k, i), ta in zip(reqs, arrivals):
             wait_until(t0, ta)
             s = sets[k]
-            ibp.decompress_fetch(s["comp"], s["mask"], s["bitval"], s["bitmask"], torch.device("cuda"),
-                                 s["comp_len"], torch.from_numpy(i).cuda())
+            pouts.append(ibp.decompress_fetch(s["comp"], s["mask"], s["bitval"], s["bitmask"], torch.device("cuda"),
+                                              s["comp_len"], torch.from_numpy(i).cuda()))
             torch.cuda.synchronize()
             plat.append((time.perf_counter() - t0 - ta) * 1e3)
         plain_wall = time.perf_counter() - t0 - arrivals[0]
+        p_ok = all(bool(torch.equal(o, sets[k]["x"][torch.from_numpy(i).cuda()])) for (k, i), o in zip(reqs, pouts))
+        del pouts
+        total_checked += 2 * nreq
+        all_exact = all_exact and ok and p_ok
         r = {"rate_per_s": "burst" if rate == float("inf") else rate, "requests": nreq,
              "served": {"p50_ms": round(float(np.percentile(lat, 50)), 3),
                         "p99_ms": round(float(np.percentile(lat, 99)), 3),
@@ -560,17 +742,21 @@ def run_serve(args):
                         "slowest_ms": {"admit": round((worst[1] - worst[0]) / 1e3, 3),
                                        "to_launch": round((worst[2] - worst[1]) / 1e3, 3),
                                        "issue": round((worst[3] - worst[2]) / 1e3, 3),
-                                       "to_done": round((worst[4] - worst[3]) / 1e3, 3)}},
+                                       "to_done": round((worst[4] - worst[3]) / 1e3, 3)},
+                        "bitexact": ok},
              "plain": {"p50_ms": round(float(np.percentile(plat, 50)), 3),
                        "p99_ms": round(float(np.percentile(plat, 99)), 3),
-                       "gbps": round(dec / plain_wall / 1e9, 2)},
-             "bitexact_sampled": ok}
+                       "gbps": round(dec / plain_wall / 1e9, 2), "bitexact": p_ok},
+             "offered_gbps": None if rate == float("inf") else round(dec / (arrivals[-1] - arrivals[0]) / 1e9, 2)}
         r["served_vs_plain_gbps"] = round(r["served"]["gbps"] / r["plain"]["gbps"], 3)
         log("serve rate %g: %s" % (rate, r))
         results.append(r)
     st = srv.stats()
     srv.close()
-    return {"workload": "serving: Poisson arrivals over 3 resident sets", "rates": results,
+    return {"workload": "serving: Poisson arrivals over %d resident sets of %d workloads, n ~ logU[1e3, 2.5e5]"
+                        % (len(sets), len(groups)),
+            "latency": "scheduled arrival -> completion, both forms", "rates": results,
+            "results_checked": total_checked, "bitexact": all_exact,
             "server_stats": {k: (round(v, 2) if isinstance(v, float) else v) for k, v in st.items()}}
 
 
@@ -612,34 +798,165 @@ def run_pcie_rows(args):
     return res
 
 
-def run_gnn_features(args):
+def run_gnn_features(args, calls=64):
     """GNN features (65536 x 128 fp32 as int64 [65536 x 64], N(0, 0.1) with 30%
-    zeros; preprocess at 0.9): one decompress_fetch of 4096 random rows per
-    call -- a latency-bound call.  Reported: per-call wall latency (issue ->
-    rows ready) from HBM and from pinned host memory, and the GPU time of a
-    call (HIP events over back-to-back calls)."""
+    zeros; preprocess at 0.9; compress_inplace in pinned host memory): calls
+    of 4096 random rows, latency-bound.  For the table in HBM and in pinned
+    memory: the isolated call's wall latency, `calls` back-to-back plain
+    calls (HIP events), the same `calls` index sets fetched by ONE
+    graph-replayed batched launch, and the host time of one call (Python
+    checks + ctypes, no GPU wait).  Every output is checked."""
     import invariantbitpacking_amd as ibp
     from invariantbitpacking_amd import workloads
-    x, idx = workloads.gnn_features()
+    x, _ = workloads.gnn_features()
     xd = x.cuda()
     mask, bitval = ibp.preprocess(xd, 0.9)
-    comp = xd.clone()
-    bm = ibp.compress_inplace(comp, mask, bitval)
+    pinned = x.clone().pin_memory()
+    bm = ibp.compress_inplace(pinned, mask, bitval)
     cl = ibp.get_single_compress_len(xd, mask, bitval)
-    pinned = comp.cpu().pin_memory()
-    idx_d = idx.cuda()
-    out = torch.empty((idx.numel(), x.shape[1]), dtype=torch.int64, device="cuda")
-    want = xd[idx_d]
-    res = {"workload": "gnn_features: 65536 x 64 int64 (fp32 [65536 x 128]), 4096 random rows per call",
-           "out_bytes_per_call": out.numel() * 8}
+    comp = pinned.cuda()
+    n, v = x.shape
+    g = torch.Generator().manual_seed(1)
+    idx = [torch.randint(0, n, (4096,), generator=g).cuda() for _ in range(calls)]
+    outs = [torch.empty((4096, v), dtype=torch.int64, device="cuda") for _ in range(calls)]
+    want = [xd[i] for i in idx]
+    dev = torch.device("cuda")
+    call_bytes = 4096 * v * 8
+    res = {"workload": "gnn_features: 65536 x 64 int64 (fp32 [65536 x 128]), calls of 4096 random rows",
+           "out_bytes_per_call": call_bytes, "calls": calls}
     for where, src in (("hbm", comp), ("pinned", pinned)):
-        f = lambda: ibp.decompress_fetch(src, mask, bitval, bm, torch.device("cuda"), cl, idx_d, None, None, None, out)
-        gpu_ms, _ = timed_region(f, args.steps, args.warmup)
-        lat = call_latencies(f, max(50, args.steps))
+        step = [0]
+
+        def one():
+            k = step[0] % calls
+            ibp.decompress_fetch(src, mask, bitval, bm, dev, cl, idx[k], None, None, None, outs[k])
+            step[0] += 1
+        lat = call_latencies(one, max(50, args.steps))
+        p_ms, _ = timed_region(lambda: [one() for _ in range(calls)], max(3, args.steps // 4), 1)
+        p_exact = all(bool(torch.equal(o, w)) for o, w in zip(outs, want))
+        # host cost of one call: issue `calls` calls without waiting for the GPU
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(calls):
+            one()
+        host_us = (time.perf_counter() - t0) / calls * 1e6
+        torch.cuda.synchronize()
+        for o in outs:
+            o.fill_(-1)
+        bf = ibp.BatchedFetch([dict(comp=src, mask=mask, bitval=bitval, bitmask=bm, index=i, out=o, comp_len=cl)
+                               for i, o in zip(idx, outs)])
+        bf.autotune()
+        bf.capture_graph()
+        for o in outs:
+            o.fill_(-1)
+        b_ms, _ = timed_region(bf.graph_launch, max(3, args.steps // 4), 1)
+        b_exact = all(bool(torch.equal(o, w)) for o, w in zip(outs, want))
+        bf.close()
         res[where] = {"p50_ms": round(float(np.percentile(lat, 50)), 4), "p99_ms": round(float(np.percentile(lat, 99)), 4),
-                      "gpu_ms_per_call": round(gpu_ms, 4), "gbps_decompressed": round(out.numel() * 8 / gpu_ms / 1e6, 1),
-                      "bitexact": bool(torch.equal(out, want))}
+                      "plain_ms_per_call": round(p_ms / calls, 5),
+                      "plain_gbps": round(calls * call_bytes / (p_ms / 1e3) / 1e9, 1),
+                      "host_us_per_call": round(host_us, 2),
+                      "batched_ms_per_%d_calls" % calls: round(b_ms, 4),
+                      "batched_gbps": round(calls * call_bytes / (b_ms / 1e3) / 1e9, 1),
+                      "batched_vs_plain": round(p_ms / b_ms, 3), "bitexact": p_exact and b_exact}
         log("gnn_features %s: %s" % (where, res[where]))
+    res["set"] = dict(x=xd, comp=comp, mask=mask, bitval=bitval, bitmask=bm, comp_len=cl)
+    return res
+
+
+def run_kernels(args):
+    """One short pass over every hot kernel on the GNN minibatch table, for
+    rocprofv3 (each op `--steps` times, HIP events around each):
+    preprocess (sweep), get_compress_size, compress_inplace, the plain fetch of
+    200,000 random rows from HBM, the raw-row gather (impl 2) of the same rows
+    from the UNCOMPRESSED table, torch's x[idx] gather, a 256 MiB device copy
+    (the counter calibration for HBM), the same fetch and raw gather from
+    pinned memory, and a sequential read of a 256 MiB pinned table (every row
+    in order, impl 2: the calibration for host-read counters)."""
+    import invariantbitpacking_amd as ibp
+    from invariantbitpacking_amd import workloads
+    reps = max(1, args.steps)
+    x = workloads.gnn_minibatch(scale=args.scale, device="cuda")
+    n, v = x.shape
+    dev = torch.device("cuda")
+    res = {"workload": "kernels: gnn_minibatch %d x %d int64, %d reps per op" % (n, v, reps)}
+
+    ops = set(args.ops.split(",")) if args.ops else None
+    want = lambda name: ops is None or name in ops  # noqa: E731
+
+    def ev(name, fn, nbytes=None):
+        if ops is not None and name not in ops:
+            return
+        fn()  # first call: kernel load, allocations
+        torch.cuda.synchronize()
+        ms, _ = timed_region(fn, reps, 0)
+        res[name] = {"ms": round(ms, 5)}
+        if nbytes:
+            res[name]["gbps"] = round(nbytes / (ms / 1e3) / 1e9, 1)
+        log("%s: %s" % (name, res[name]))
+
+    st = {}
+
+    def pre():
+        st["mb"] = ibp.preprocess(x, None)
+    ev("preprocess", pre, x.numel() * 8)
+    mask, bitval = st["mb"] if "mb" in st else ibp.preprocess(x, None)
+    ev("get_compress_size", lambda: ibp.get_compress_size(x, mask, bitval), x.numel() * 8)
+    comp = x.clone()
+    src = x.clone()
+
+    def pack():
+        comp.copy_(src)
+        st["bm"] = ibp.compress_inplace(comp, mask, bitval)
+    ev("copy_and_compress_inplace", pack)
+    if "bm" not in st:
+        st["bm"] = ibp.compress_inplace(comp, mask, bitval)
+    bm = st["bm"]
+    cl = ibp.get_single_compress_len(x, mask, bitval)
+    idx = workloads.gnn_minibatch_indices(n, min(args.batch, n), 0).cuda()
+    out = torch.empty((idx.numel(), v), dtype=torch.int64, device="cuda")
+    dec = out.numel() * 8
+    ev("fetch_hbm", lambda: ibp.decompress_fetch(comp, mask, bitval, bm, dev, cl, idx, None, None, None, out), dec)
+    ok = ops is None or "fetch_hbm" not in ops or bool(torch.equal(out, x[idx]))
+    zero = torch.zeros_like(bm)
+    ev("raw_gather_hbm", lambda: ibp.decompress_fetch(x, mask, bitval, zero, dev, v, idx, None, None, 2, out), dec)
+    ok = ok and (ops is None or "raw_gather_hbm" not in ops or bool(torch.equal(out, x[idx])))
+    ev("torch_index_gather_hbm", lambda: torch.index_select(x, 0, idx, out=out), dec)
+    if want("device_copy_256MiB"):
+        a = torch.empty(256 << 20, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        ev("device_copy_256MiB", lambda: b.copy_(a), 2 * a.numel())
+        del a, b
+    if want("seq_copy_hbm"):
+        # the raw-row kernel over every row in order: a plain copy of the
+        # whole table (n*V*8 bytes read and written), the calibration of the
+        # HBM counters for the fetch kernel's family
+        whole = torch.empty_like(x)
+        ev("seq_copy_hbm", lambda: ibp.decompress_fetch(x, mask, bitval, zero, dev, v, None, None, None, 2, whole),
+           2 * x.numel() * 8)
+        ok = ok and bool(torch.equal(whole, x))
+        del whole
+    if not args.no_pcie and want("fetch_pinned"):
+        comp_h = comp.cpu().pin_memory()
+        ev("fetch_pinned", lambda: ibp.decompress_fetch(comp_h, mask, bitval, bm, dev, cl, idx, None, None, None,
+                                                        out), dec)
+        ok = ok and bool(torch.equal(out, x[idx]))
+        del comp_h
+    if not args.no_pcie and want("raw_gather_pinned"):
+        x_h = x.cpu().pin_memory()
+        ev("raw_gather_pinned", lambda: ibp.decompress_fetch(x_h, mask, bitval, zero, dev, v, idx, None, None, 2,
+                                                             out), dec)
+        ok = ok and bool(torch.equal(out, x[idx]))
+        del x_h
+    if not args.no_pcie and want("sequential_read_pinned_256MiB"):
+        rows = (256 << 20) // 512
+        seq_h = torch.zeros((rows, 64), dtype=torch.int64).pin_memory()
+        seq_d = torch.empty((rows, 64), dtype=torch.int64, device="cuda")
+        z64 = torch.zeros(64, dtype=torch.int64, device="cuda")
+        zb = torch.zeros((rows + 31) // 32, dtype=torch.int32, device="cuda")
+        ev("sequential_read_pinned_256MiB", lambda: ibp.decompress_fetch(seq_h, z64, z64, zb, dev, 64, None, None,
+                                                                         None, 2, seq_d), rows * 512)
+    res["bitexact"] = ok
     return res
 
 
@@ -755,6 +1072,39 @@ def run_llm(args):
             "bitexact": exact and d_exact}
 
 
+def run_default(args):
+    """The headline line (GNN minibatch) with the other workloads as legs:
+    GNN features (4096-row calls, plain vs batched), DLRM (26 tables, plain vs
+    one launch), serving (Poisson mix over the GNN features, GNN minibatch and
+    DLRM sets) and LLM offload (one decoder layer, k-means masks)."""
+    import copy
+    traffic = None if args.no_traffic else measure_traffic(args)
+    res = run_gnn_minibatch(args, traffic)
+    g2 = res.pop("set")
+    if args.no_legs:
+        return res
+    legs = {}
+    f = run_gnn_features(args)
+    g1 = f.pop("set")
+    legs["gnn_features"] = f
+    da = copy.copy(args)
+    da.batch = 65536
+    d = run_dlrm(da, condensed=False)
+    g3 = d.pop("sets")
+    legs["dlrm"] = d
+    legs["serve"] = run_serve(args, groups=[[g1], [g2], g3])
+    del g1, g2, g3
+    torch.cuda.empty_cache()
+    la = copy.copy(args)
+    la.layers = 1
+    legs["llm"] = run_llm(la)
+    res["legs"] = legs
+    res["bitexact_all_legs"] = res["bitexact"] and all(
+        leg.get("bitexact", True) and all(v.get("bitexact", True) for v in leg.values() if isinstance(v, dict))
+        for leg in legs.values())
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -767,34 +1117,44 @@ def main():
     ap.add_argument("--no-pcie", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--workload", default="gnn_minibatch",
-                    choices=["gnn_minibatch", "gnn_features", "dlrm", "serve", "llm", "pcie_rows"],
+                    choices=["gnn_minibatch", "gnn_features", "dlrm", "serve", "llm", "pcie_rows", "kernels"],
                     help="gnn_minibatch is the headline line; the others print their own JSON")
     ap.add_argument("--rates", type=float, nargs="+", default=[1e2, 1e3, 1e4])
-    ap.add_argument("--layers", type=int, default=2, help="decoder layers of the llm workload")
-    ap.add_argument("--calibrate-copy", action="store_true",
-                    help="also run 4 device copies of 256 MiB (the reference kernel for HBM counter scaling)")
+    ap.add_argument("--layers", type=int, default=32, help="decoder layers of the llm workload")
+    ap.add_argument("--ops", default="", help="kernels workload: comma-separated subset of its ops")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter passes of the default run")
+    ap.add_argument("--no-legs", action="store_true",
+                    help="default run: only the GNN minibatch line, without the other workloads' legs")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(rows shard per GPU over each GPU's own PCIe link; see the design notes)", file=sys.stderr)
         sys.exit(2)
-    if not torch.cuda.is_available():
+    # device_count() does not initialise the GPU: the counter passes of the
+    # default run start child processes before this process touches it.
+    if torch.cuda.device_count() < 1:
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
     if args.workload == "dlrm":
         if args.batch == 200000:
             args.batch = 65536
-        print(json.dumps(run_dlrm(args)))
+        r = run_dlrm(args)
+        r.pop("sets")
+        print(json.dumps(r))
     elif args.workload == "serve":
         print(json.dumps(run_serve(args)))
     elif args.workload == "llm":
         print(json.dumps(run_llm(args)))
     elif args.workload == "gnn_features":
-        print(json.dumps(run_gnn_features(args)))
+        r = run_gnn_features(args)
+        r.pop("set")
+        print(json.dumps(r))
     elif args.workload == "pcie_rows":
         print(json.dumps(run_pcie_rows(args)))
+    elif args.workload == "kernels":
+        print(json.dumps(run_kernels(args)))
     else:
-        print(json.dumps(run_gnn_minibatch(args)))
+        print(json.dumps(run_default(args)))
 
 
 if __name__ == "__main__":
