@@ -901,6 +901,8 @@ def run_kernels(args):
         st["mb"] = ibp.preprocess(x, None)
     ev("preprocess", pre, x.numel() * 8)
     mask, bitval = st["mb"] if "mb" in st else ibp.preprocess(x, None)
+    res["mask_checksum"] = "%x:%x" % (int(mask.view(torch.int64).sum()) & 0xffffffffffff,
+                                      int(bitval.view(torch.int64).sum()) & 0xffffffffffff)
     ev("get_compress_size", lambda: ibp.get_compress_size(x, mask, bitval), x.numel() * 8)
     comp = x.clone()
     src = x.clone()
@@ -912,6 +914,8 @@ def run_kernels(args):
     if "bm" not in st:
         st["bm"] = ibp.compress_inplace(comp, mask, bitval)
     bm = st["bm"]
+    res["pack_checksum"] = "%x:%x" % (int(comp.view(-1)[::7].sum()) & 0xffffffffffff,
+                                      int(bm.to(torch.int64).sum()) & 0xffffffffffff)
     cl = ibp.get_single_compress_len(x, mask, bitval)
     idx = workloads.gnn_minibatch_indices(n, min(args.batch, n), 0).cuda()
     out = torch.empty((idx.numel(), v), dtype=torch.int64, device="cuda")

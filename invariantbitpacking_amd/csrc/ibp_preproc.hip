@@ -125,16 +125,22 @@ __global__ __launch_bounds__(1024) void reduce_counts_kernel(const int32_t *part
 // row-per-lane score kernel reads with scalar loads.
 template <typename T>
 __global__ void create_masks_kernel(const int32_t *cnt, int64_t num_vecs, int64_t V, Thresholds th, T *masks,
-                                    T *bitvals, T *colm, T *colb)
+                                    T *bitvals, T *colm, T *colb, uint32_t *colp)
 {
     constexpr int B = 8 * (int)sizeof(T);
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid < kMaxThresholds) { // the zero column V after the last
+        colm[V * kMaxThresholds + gid] = T(0);
+        colb[V * kMaxThresholds + gid] = T(0);
+        colp[V * kMaxThresholds + gid] = 0;
+    }
     if (gid >= V * kMaxThresholds) return;
     const int t = (int)(gid / V);
     const int64_t j = gid % V;
     if (t >= th.count) { // unused thresholds: zero masks in the column-major copy
         colm[j * kMaxThresholds + t] = T(0);
         colb[j * kMaxThresholds + t] = T(0);
+        colp[j * kMaxThresholds + t] = 0;
         return;
     }
     const float thr = th.t[t];
@@ -154,6 +160,7 @@ __global__ void create_masks_kernel(const int32_t *cnt, int64_t num_vecs, int64_
     bitvals[gid] = (T)val;
     colm[j * kMaxThresholds + t] = (T)masker;
     colb[j * kMaxThresholds + t] = (T)val;
+    colp[j * kMaxThresholds + t] = (uint32_t)__builtin_popcountll((uint64_t)(T)masker);
 }
 
 // scores[t] += sum over rows of max(0, S_t(row) - V), S_t = bits of the
@@ -263,25 +270,40 @@ __device__ __forceinline__ uint32_t any_bit(T u)
 
 // lost + (d & m != 0 ? popc(m) : 0) as AND, (AND-OR), MIN, MAD_U24 with the
 // mask and its popcount in SGPRs (one scalar operand per instruction).
+// ORDERED keeps the steps in program order with the other volatile asm of the
+// loop, so the scalar loads of the next column stay ahead of them and their
+// wait behind them.
+#define IBP_LOST_STEP64                                                                                                \
+    "v_and_b32 %0, %2, %3\n\t"                                                                                          \
+    "v_and_or_b32 %0, %4, %5, %0\n\t"                                                                                   \
+    "v_min_u32 %0, 1, %0\n\t"                                                                                           \
+    "v_mad_u32_u24 %1, %0, %6, %1"
+#define IBP_LOST_STEP32                                                                                                \
+    "v_and_b32 %0, %2, %3\n\t"                                                                                          \
+    "v_min_u32 %0, 1, %0\n\t"                                                                                           \
+    "v_mad_u32_u24 %1, %0, %4, %1"
+template <bool ORDERED = false>
 __device__ __forceinline__ uint32_t lost_step(uint64_t d, uint64_t m, uint32_t pc, uint32_t lost)
 {
     uint32_t z;
-    asm("v_and_b32 %0, %2, %3\n\t"
-        "v_and_or_b32 %0, %4, %5, %0\n\t"
-        "v_min_u32 %0, 1, %0\n\t"
-        "v_mad_u32_u24 %1, %0, %6, %1"
-        : "=&v"(z), "+v"(lost)
-        : "s"((uint32_t)m), "v"((uint32_t)d), "v"((uint32_t)(d >> 32)), "s"((uint32_t)(m >> 32)), "s"(pc));
+    if constexpr (ORDERED)
+        asm volatile(IBP_LOST_STEP64
+                     : "=&v"(z), "+v"(lost)
+                     : "s"((uint32_t)m), "v"((uint32_t)d), "v"((uint32_t)(d >> 32)), "s"((uint32_t)(m >> 32)), "s"(pc));
+    else
+        asm(IBP_LOST_STEP64
+            : "=&v"(z), "+v"(lost)
+            : "s"((uint32_t)m), "v"((uint32_t)d), "v"((uint32_t)(d >> 32)), "s"((uint32_t)(m >> 32)), "s"(pc));
     return lost;
 }
+template <bool ORDERED = false>
 __device__ __forceinline__ uint32_t lost_step(uint32_t d, uint32_t m, uint32_t pc, uint32_t lost)
 {
     uint32_t z;
-    asm("v_and_b32 %0, %2, %3\n\t"
-        "v_min_u32 %0, 1, %0\n\t"
-        "v_mad_u32_u24 %1, %0, %4, %1"
-        : "=&v"(z), "+v"(lost)
-        : "s"(m), "v"(d), "s"(pc));
+    if constexpr (ORDERED)
+        asm volatile(IBP_LOST_STEP32 : "=&v"(z), "+v"(lost) : "s"(m), "v"(d), "s"(pc));
+    else
+        asm(IBP_LOST_STEP32 : "=&v"(z), "+v"(lost) : "s"(m), "v"(d), "s"(pc));
     return lost;
 }
 
@@ -369,18 +391,24 @@ __global__ __launch_bounds__(256) void score_rows_kernel(const T *__restrict__ d
     if ((int)threadIdx.x < nt && threadIdx.x < NT && bsum[threadIdx.x]) atomicAdd(&scores[threadIdx.x], bsum[threadIdx.x]);
 }
 
-// Row-per-lane scoring from an LDS copy of a 64-row tile.  A tile's rows are
-// contiguous in memory, so a wave copies it into its LDS with C = row bytes
-// / 16 instructions of global_load_lds_dwordx4, each moving 1 KB of
-// consecutive bytes straight to LDS (no registers; the copy is tracked by
-// vmcnt).  Lane r then reads row r from LDS chunk by chunk, with the column
-// masks uniform scalar loads as in score_rows_kernel.  (Loading row r straight
-// into lane r makes every load instruction touch 64 rows 400 bytes apart, so
-// each 128-byte line is fetched again by later loads.)  Rows sit at their
-// natural stride; the kernel is used where that stride in dwords is not a
-// multiple of 16 modulo 32, so eight lanes reading 16 bytes each hit at most
-// two bank groups twice.  One wave per block, each with its own tile;
-// __syncthreads() orders the wave's copy and reads.
+// Row-per-lane scoring from LDS copies of 64-row tiles.  A tile's rows are
+// contiguous in memory, so it is copied to LDS with C = row bytes / 16
+// instructions of global_load_lds_dwordx4, each moving 1 KB of consecutive
+// bytes straight to LDS (no registers; tracked by vmcnt).  Lane r then reads
+// row r from LDS chunk by chunk, with the column masks uniform scalar loads.
+// (Loading row r straight into lane r makes every load instruction touch 64
+// rows 400 bytes apart, so each 128-byte line is fetched again by later
+// loads.)
+//
+// The WPT waves of a block share the tile and split the thresholds, TPW
+// each: a tile holds 64 rows whatever the number of waves scoring it, so
+// four waves per tile give four times the waves per CU that one wave per
+// tile would, and each does a quarter of the vector work (GNN table, 7
+// thresholds: 237 us at two per wave, 280 at four, 309 at seven).  With DB
+// the block keeps two tile buffers and issues the next tile's copy before
+// scoring the current one; that halves the blocks per CU and is slower.
+// Used where the row stride in dwords is not a multiple of 16 modulo 32, so
+// eight lanes reading 16 bytes each hit at most two bank groups twice.
 constexpr int kTileMaxRowBytes = 512;
 __host__ __device__ constexpr bool tile_rows_ok(int64_t row_bytes)
 {
@@ -388,113 +416,306 @@ __host__ __device__ constexpr bool tile_rows_ok(int64_t row_bytes)
            (row_bytes / 4) % 32 != 16;
 }
 
-// Masks (NT) and bitvals (NB: one when the thresholds are nested) of column
-// j from the column-major tables, as uniform scalar loads.
-template <typename T, int NT, int NB>
-__device__ __forceinline__ void load_cols(const T *__restrict__ colm, const T *__restrict__ colb, int64_t j, T (&m)[NT],
-                                          T (&b)[NB])
+// The column constants one wave of score_tile_kernel needs for one 16-byte
+// chunk of a row (EPC elements): masks and their popcounts for its TPW
+// thresholds, and the bitvals (one per element when the thresholds are
+// nested, else one per threshold).  Records are laid out [wave][chunk], with a
+// zero record after the last chunk, so a chunk's constants are three uniform
+// scalar loads from one pointer.
+template <typename T, int TPW, int NB>
+struct ScoreRec {
+    static constexpr int EPC = 16 / (int)sizeof(T);
+    T m[EPC][TPW];
+    uint32_t p[EPC][TPW];
+    T b[EPC][NB];
+};
+
+// Fills the records of WPT waves for C chunks (plus the zero record) from the
+// column-major tables; thread = (wave, chunk, element, threshold slot).
+template <typename T, int TPW, int NB>
+__global__ void score_recs_kernel(const T *__restrict__ colm, const T *__restrict__ colb,
+                                  const uint32_t *__restrict__ colp, int64_t V, int C, int WPT,
+                                  ScoreRec<T, TPW, NB> *recs)
 {
-#pragma unroll
-    for (int t = 0; t < NT; ++t) m[t] = colm[j * kMaxThresholds + t];
-#pragma unroll
-    for (int t = 0; t < NB; ++t) b[t] = colb[j * kMaxThresholds + t];
+    constexpr int EPC = ScoreRec<T, TPW, NB>::EPC;
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= (int64_t)WPT * (C + 1) * EPC * TPW) return;
+    const int t = (int)(gid % TPW);
+    const int e = (int)((gid / TPW) % EPC);
+    const int64_t c = (gid / ((int64_t)TPW * EPC)) % (C + 1);
+    const int w = (int)(gid / ((int64_t)TPW * EPC * (C + 1)));
+    const int64_t j = c * EPC + e;
+    const int tt = w * TPW + t;
+    const bool ok = j < V && tt < kMaxThresholds;
+    ScoreRec<T, TPW, NB> &r = recs[(int64_t)w * (C + 1) + c];
+    r.m[e][t] = ok ? colm[j * kMaxThresholds + tt] : T(0);
+    r.p[e][t] = ok ? colp[j * kMaxThresholds + tt] : 0u;
+    if (NB == 1) {
+        if (t == 0) r.b[e][0] = j < V ? colb[j * kMaxThresholds] : T(0);
+    } else {
+        r.b[e][NB == 1 ? 0 : t] = ok ? colb[j * kMaxThresholds + tt] : T(0);
+    }
 }
 
-template <typename T, int NT, bool NESTED>
-__global__ __launch_bounds__(64) void score_tile_kernel(const T *__restrict__ data, int64_t num_vecs, int V,
-                                                        const T *__restrict__ colm, const T *__restrict__ colb,
-                                                        int nt, unsigned long long *scores)
+// Loads one record as plain element loads from the kernel's read-only
+// argument (a struct copy would be lowered to vector loads).
+template <typename T, int TPW, int NB>
+__device__ __forceinline__ void load_rec(const ScoreRec<T, TPW, NB> *__restrict__ r, T (&m)[ScoreRec<T, TPW, NB>::EPC][TPW],
+                                         uint32_t (&p)[ScoreRec<T, TPW, NB>::EPC][TPW],
+                                         T (&b)[ScoreRec<T, TPW, NB>::EPC][NB])
+{
+    constexpr int EPC = ScoreRec<T, TPW, NB>::EPC;
+    const T *rm = reinterpret_cast<const T *>(r);
+    const uint32_t *rp = reinterpret_cast<const uint32_t *>(rm + EPC * TPW);
+    const T *rb = reinterpret_cast<const T *>(rp + EPC * TPW);
+#pragma unroll
+    for (int e = 0; e < EPC; ++e)
+#pragma unroll
+        for (int t = 0; t < TPW; ++t) m[e][t] = rm[e * TPW + t];
+#pragma unroll
+    for (int e = 0; e < EPC; ++e)
+#pragma unroll
+        for (int t = 0; t < TPW; ++t) p[e][t] = rp[e * TPW + t];
+#pragma unroll
+    for (int e = 0; e < EPC; ++e)
+#pragma unroll
+        for (int t = 0; t < NB; ++t) b[e][t] = rb[e * NB + t];
+}
+
+// x ^ b (b uniform) as volatile asm, so it stays in order with the steps.
+__device__ __forceinline__ uint64_t xor_ordered(uint64_t x, uint64_t b)
+{
+    uint32_t lo, hi;
+    asm volatile("v_xor_b32 %0, %2, %3\n\tv_xor_b32 %1, %4, %5"
+                 : "=&v"(lo), "=&v"(hi)
+                 : "s"((uint32_t)b), "v"((uint32_t)x), "s"((uint32_t)(b >> 32)), "v"((uint32_t)(x >> 32)));
+    return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ uint32_t xor_ordered(uint32_t x, uint32_t b)
+{
+    uint32_t d;
+    asm volatile("v_xor_b32 %0, %1, %2" : "=v"(d) : "s"(b), "v"(x));
+    return d;
+}
+
+// Empty asm that needs the record's values in SGPRs: the wait for their
+// scalar loads is placed here.
+template <typename T, int TPW, int NB>
+__device__ __forceinline__ void need_rec(const T (&m)[ScoreRec<T, TPW, NB>::EPC][TPW],
+                                         const uint32_t (&p)[ScoreRec<T, TPW, NB>::EPC][TPW],
+                                         const T (&b)[ScoreRec<T, TPW, NB>::EPC][NB])
+{
+    constexpr int EPC = ScoreRec<T, TPW, NB>::EPC;
+#pragma unroll
+    for (int e = 0; e < EPC; ++e) {
+#pragma unroll
+        for (int t = 0; t < TPW; ++t) asm volatile("" ::"s"(m[e][t]), "s"(p[e][t]));
+#pragma unroll
+        for (int t = 0; t < NB; ++t) asm volatile("" ::"s"(b[e][t]));
+    }
+}
+
+// The copy of tile tl into `buf` by wave `wib` of WPT: chunks wib, wib + WPT, ...
+template <typename T, int WPT>
+__device__ __forceinline__ void copy_tile(const T *data, int64_t num_vecs, int V, int C, int64_t tl, int wib, int lane,
+                                          unsigned char *buf)
+{
+    const uint4 *src = reinterpret_cast<const uint4 *>(data + tl * kWave * (int64_t)V);
+    const int64_t rows_left = num_vecs - tl * kWave;
+    // the last tile may be short: its missing rows re-read the last chunk
+    const int64_t last = (rows_left < kWave ? rows_left : kWave) * C - 1;
+    for (int k = wib; k < C; k += WPT) {
+        const int64_t g = (int64_t)k * kWave + lane;
+        __builtin_amdgcn_global_load_lds(static_cast<const void *>(src + (g < last ? g : last)),
+                                         (__attribute__((address_space(3))) void *)(buf + (size_t)k * (kWave * 16)),
+                                         16, 0, 0);
+    }
+}
+
+// TPW thresholds per wave (t0 = wave * TPW), WPT waves per tile; DB: two
+// tile buffers per block, the next tile's copy issued before the current
+// one is scored.  Every threshold of a wave is scored without a branch (a
+// zero mask loses nothing), with the mask popcounts from the records.
+template <typename T, int TPW, int WPT, bool DB, bool NESTED>
+__global__ __launch_bounds__(64 * WPT) void score_tile_kernel(const T *__restrict__ data, int64_t num_vecs, int V,
+                                                              const ScoreRec<T, TPW, NESTED ? 1 : TPW> *__restrict__ recs,
+                                                              const uint32_t *__restrict__ colp, int nt,
+                                                              unsigned long long *scores)
 {
     constexpr int EPC = 16 / (int)sizeof(T);
-    constexpr int NB = NESTED ? 1 : NT;
-    extern __shared__ __align__(16) unsigned char tile[];
+    constexpr int NB = NESTED ? 1 : TPW;
+    extern __shared__ __align__(16) unsigned char tiles[];
     const int lane = lane_id();
-    const int C = V * (int)sizeof(T) / 16; // chunks per row = copy instructions per tile
+    const int wib = wave_in_block_id();
+    const int t0 = wib * TPW; // this wave's first threshold
+    const int C = V * (int)sizeof(T) / 16;
+    const size_t tile_bytes = (size_t)C * kWave * 16;
     const int64_t ntiles = (num_vecs + kWave - 1) / kWave;
-    const int64_t nwaves = gridDim.x;
-    uint32_t pcsum[NT];
-    unsigned long long saved[NT];
+    const int64_t nblocks = gridDim.x;
+    const ScoreRec<T, TPW, NB> *rec = recs + (size_t)wib * (C + 1);
+    uint32_t pcsum[TPW];
+    unsigned long long saved[TPW];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
+    for (int t = 0; t < TPW; ++t) {
         pcsum[t] = 0;
         saved[t] = 0;
     }
     for (int j = lane; j < V; j += kWave)
 #pragma unroll
-        for (int t = 0; t < NT; ++t)
-            if (t < nt) pcsum[t] += (uint32_t)popc((WideT<T>)colm[(int64_t)j * kMaxThresholds + t]);
+        for (int t = 0; t < TPW; ++t)
+            if (t0 + t < nt) pcsum[t] += colp[(int64_t)j * kMaxThresholds + t0 + t];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) pcsum[t] = (uint32_t)wave_sum((int)pcsum[t]);
-    const uint4 *mine = reinterpret_cast<const uint4 *>(tile) + lane * C;
-    for (int64_t tl = blockIdx.x; tl < ntiles; tl += nwaves) {
-        const uint4 *src = reinterpret_cast<const uint4 *>(data + tl * kWave * (int64_t)V);
-        const int64_t rows_left = num_vecs - tl * kWave;
-        // the last tile may be short: its missing rows re-read the last chunk
-        const int64_t last = (rows_left < kWave ? rows_left : kWave) * C - 1;
-        for (int k = 0; k < C; ++k) {
-            const int64_t g = (int64_t)k * kWave + lane;
-            __builtin_amdgcn_global_load_lds(static_cast<const void *>(src + (g < last ? g : last)),
-                                             (__attribute__((address_space(3))) void *)(tile + (size_t)k * (kWave * 16)),
-                                             16, 0, 0);
-        }
-        __builtin_amdgcn_s_waitcnt(0x0f70); // vmcnt(0): the copy has landed in LDS
-        __syncthreads();
-        const int64_t row = tl * kWave + lane;
-        uint32_t lost[NT];
+    for (int t = 0; t < TPW; ++t) pcsum[t] = (uint32_t)wave_sum((int)pcsum[t]);
+    const bool active = t0 < nt; // waves past the last threshold only copy
+    int cur = 0;
+    int64_t tl = blockIdx.x;
+    if (DB && tl < ntiles) copy_tile<T, WPT>(data, num_vecs, V, C, tl, wib, lane, tiles);
+    for (; tl < ntiles; tl += nblocks) {
+        if (!DB) copy_tile<T, WPT>(data, num_vecs, V, C, tl, wib, lane, tiles);
+        __builtin_amdgcn_s_waitcnt(0x0f70); // vmcnt(0): this wave's part of the tile has landed
+        __syncthreads();                    // ... and every other wave's part
+        const unsigned char *buf = tiles + (size_t)cur * tile_bytes;
+        // the next tile goes to the other buffer, read by the previous
+        // iteration (finished at the barrier above)
+        if (DB && tl + nblocks < ntiles)
+            copy_tile<T, WPT>(data, num_vecs, V, C, tl + nblocks, wib, lane, tiles + (size_t)(cur ^ 1) * tile_bytes);
+        if (active) {
+            const uint4 *mine = reinterpret_cast<const uint4 *>(buf) + lane * C;
+            const int64_t row = tl * kWave + lane;
+            uint32_t lost[TPW];
 #pragma unroll
-        for (int t = 0; t < NT; ++t) lost[t] = 0;
-        // The column constants of element j + 1 are loaded while element j
-        // is scored: scalar loads complete in any order, so their only wait
-        // is lgkmcnt(0), which the empty asm statements place before the next
-        // loads are issued rather than after.
-        T mc[NT], bc[NB];
-        load_cols<T, NT, NB>(colm, colb, 0, mc, bc);
-        uint4 v = mine[0];
-        for (int c = 0; c < C; ++c) {
+            for (int t = 0; t < TPW; ++t) lost[t] = 0;
+            // Chunk c + 1's constants (three scalar loads) and its 16 bytes
+            // of the row (one LDS read) are issued before chunk c is scored.
+            // Scalar loads complete in any order, so their only wait is
+            // lgkmcnt(0): the empty asm statements of need_rec place it at
+            // the start of chunk c + 1, and the ordered steps keep chunk c's
+            // work ahead of it.  Two register sets alternate, so nothing is
+            // moved between chunks.
+            T ma[EPC][TPW], mb[EPC][TPW], ba[EPC][NB], bb[EPC][NB];
+            uint32_t pa[EPC][TPW], pb[EPC][TPW];
+            auto score = [&](const uint4 &v, const T(&m)[EPC][TPW], const uint32_t(&pc)[EPC][TPW],
+                             const T(&bv)[EPC][NB]) {
 #pragma unroll
-            for (int e = 0; e < EPC; ++e) {
-                const int64_t j = (int64_t)c * EPC + e;
+                for (int e = 0; e < EPC; ++e) {
+                    const T x = chunk_elem<T>(v, e);
+                    const T d0 = xor_ordered(x, bv[e][0]);
 #pragma unroll
-                for (int t = 0; t < NT; ++t) asm volatile("" ::"s"(mc[t]));
-#pragma unroll
-                for (int t = 0; t < NB; ++t) asm volatile("" ::"s"(bc[t]));
-                asm volatile("" ::"v"(v.x), "v"(v.y), "v"(v.z), "v"(v.w) : "memory");
-                const T x = chunk_elem<T>(v, e);
-                T mn[NT], bn[NB];
-                load_cols<T, NT, NB>(colm, colb, j + 1 < V ? j + 1 : j, mn, bn);
-                if (e == EPC - 1 && c + 1 < C) v = mine[c + 1];
-                const T d0 = (T)(x ^ bc[0]);
-#pragma unroll
-                for (int t = 0; t < NT; ++t) {
-                    if (mc[t] == 0) continue; // uniform: no bits, nothing to lose
-                    const T d = NESTED ? d0 : (T)(x ^ bc[t]);
-                    lost[t] = lost_step(d, mc[t], (uint32_t)popc((WideT<T>)mc[t]), lost[t]);
+                    for (int t = 0; t < TPW; ++t) {
+                        const T d = NESTED ? d0 : xor_ordered(x, bv[e][NB == 1 ? 0 : t]);
+                        lost[t] = lost_step<true>(d, m[e][t], pc[e][t], lost[t]);
+                    }
                 }
+            };
+            load_rec<T, TPW, NB>(rec, ma, pa, ba);
+            uint4 va = mine[0], vb;
+            int c = 0;
+            for (; c + 1 < C; c += 2) {
+                need_rec<T, TPW, NB>(ma, pa, ba);
+                asm volatile("" ::"v"(va.x), "v"(va.y), "v"(va.z), "v"(va.w));
+                load_rec<T, TPW, NB>(rec + c + 1, mb, pb, bb);
+                vb = mine[c + 1];
+                score(va, ma, pa, ba);
+                need_rec<T, TPW, NB>(mb, pb, bb);
+                asm volatile("" ::"v"(vb.x), "v"(vb.y), "v"(vb.z), "v"(vb.w));
+                load_rec<T, TPW, NB>(rec + c + 2, ma, pa, ba); // record C is the zero one
+                va = mine[c + 2 < C ? c + 2 : c + 1];
+                score(vb, mb, pb, bb);
+            }
+            if (c < C) { // odd C: the last chunk
+                need_rec<T, TPW, NB>(ma, pa, ba);
+                score(va, ma, pa, ba);
+            }
+            if (row < num_vecs) {
 #pragma unroll
-                for (int t = 0; t < NT; ++t) mc[t] = mn[t];
-#pragma unroll
-                for (int t = 0; t < NB; ++t) bc[t] = bn[t];
+                for (int t = 0; t < TPW; ++t) {
+                    const uint32_t sv = pcsum[t] - lost[t];
+                    if (t0 + t < nt && sv > (uint32_t)V) saved[t] += (unsigned long long)(sv - (uint32_t)V);
+                }
             }
         }
-        if (row < num_vecs) {
-#pragma unroll
-            for (int t = 0; t < NT; ++t) {
-                const uint32_t sv = pcsum[t] - lost[t];
-                if (t < nt && sv > (uint32_t)V) saved[t] += (unsigned long long)(sv - (uint32_t)V);
-            }
+        if (DB) {
+            cur ^= 1;
+        } else {
+            __syncthreads(); // this tile's reads are done before the next copy lands
         }
-        __syncthreads(); // this tile's reads are done before the next copy lands
     }
     // One atomic per threshold and wave: the lanes' savings are summed as
     // 24-bit low parts and the rest (each sum of 64 fits 32 bits as long as a
     // lane's saving is below 2^49).
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
+    for (int t = 0; t < TPW; ++t) {
         const uint32_t lo = (uint32_t)wave_sum((int)(uint32_t)(saved[t] & 0xffffffull));
         const uint32_t hi = (uint32_t)wave_sum((int)(uint32_t)(saved[t] >> 24));
         const unsigned long long tot = (unsigned long long)lo + ((unsigned long long)hi << 24);
-        if (lane == 0 && t < nt && tot) atomicAdd(&scores[t], tot);
+        if (lane == 0 && t0 + t < nt && tot) atomicAdd(&scores[t0 + t], tot);
     }
+}
+
+// Launch of score_tile_kernel for `nt` thresholds (1 or a sweep of up to
+// kMaxThresholds): IBP_SCORE_SWEEP_TPW thresholds per wave and as many waves
+// per tile as the sweep needs, one wave for one threshold.  The records go to
+// `recbuf` (score_recs_bytes), written by score_recs_kernel first.
+#ifndef IBP_SCORE_SWEEP_TPW
+#define IBP_SCORE_SWEEP_TPW 2
+#endif
+#ifndef IBP_SCORE_DB
+#define IBP_SCORE_DB 0
+#endif
+constexpr int kScoreTPW = IBP_SCORE_SWEEP_TPW;
+constexpr int kScoreWPT = (kMaxThresholds - 1 + kScoreTPW - 1) / kScoreTPW;
+
+template <typename T, int TPW, int WPT, bool NESTED>
+void launch_score_tile_cfg(const T *data, int64_t N, int64_t V, const T *colm, const T *colb, const uint32_t *colp,
+                           int nt, unsigned long long *scores, void *recbuf, int64_t grid, size_t lds,
+                           hipStream_t stream)
+{
+    constexpr int NB = NESTED ? 1 : TPW;
+    constexpr bool DB = IBP_SCORE_DB != 0;
+    using Rec = ScoreRec<T, TPW, NB>;
+    const int C = (int)(V * (int64_t)sizeof(T) / 16);
+    const int64_t threads = (int64_t)WPT * (C + 1) * Rec::EPC * TPW;
+    hipLaunchKernelGGL((score_recs_kernel<T, TPW, NB>), dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, stream,
+                       colm, colb, colp, V, C, WPT, (Rec *)recbuf);
+    hipLaunchKernelGGL((score_tile_kernel<T, TPW, WPT, DB, NESTED>), dim3((unsigned)grid), dim3(kWave * WPT), lds,
+                       stream, data, N, (int)V, (const Rec *)recbuf, colp, nt, scores);
+}
+
+// Bytes of records the launch below needs for rows of V elements of w bytes.
+size_t score_recs_bytes(int w, int64_t V)
+{
+    auto need = [&](auto tag) -> size_t {
+        using T = decltype(tag);
+        const int64_t C = (V * (int64_t)sizeof(T) + 15) / 16;
+        const size_t one = (size_t)(C + 1) * sizeof(ScoreRec<T, 1, 1>);
+        const size_t sweep = (size_t)kScoreWPT * (C + 1) * sizeof(ScoreRec<T, kScoreTPW, kScoreTPW>);
+        return one > sweep ? one : sweep;
+    };
+    if (w == 8) return need(uint64_t());
+    if (w == 4) return need(uint32_t());
+    return 0;
+}
+
+template <typename T>
+void launch_score_tile(const T *data, int64_t N, int64_t V, const T *colm, const T *colb, const uint32_t *colp,
+                       int nt, bool nested, unsigned long long *scores, void *recbuf, int num_cus, hipStream_t stream)
+{
+    constexpr bool DB = IBP_SCORE_DB != 0;
+    const size_t lds = (DB ? 2 : 1) * (size_t)kWave * (size_t)V * sizeof(T);
+    const int64_t ntiles = (N + kWave - 1) / kWave;
+    int64_t per_cu = (int64_t)(160 * 1024) / (int64_t)lds;
+    if (per_cu > 8) per_cu = 8;
+    if (per_cu < 1) per_cu = 1;
+    int64_t grid = (int64_t)num_cus * per_cu;
+    if (grid > ntiles) grid = ntiles;
+    if (nt <= 1)
+        launch_score_tile_cfg<T, 1, 1, true>(data, N, V, colm, colb, colp, nt, scores, recbuf, grid, lds, stream);
+    else if (nested)
+        launch_score_tile_cfg<T, kScoreTPW, kScoreWPT, true>(data, N, V, colm, colb, colp, nt, scores, recbuf, grid,
+                                                             lds, stream);
+    else
+        launch_score_tile_cfg<T, kScoreTPW, kScoreWPT, false>(data, N, V, colm, colb, colp, nt, scores, recbuf, grid,
+                                                              lds, stream);
 }
 
 // First strict maximum of the scores compared as double
@@ -554,7 +775,7 @@ bool thresholds_nested(const Thresholds &th)
 namespace {
 
 struct PreWorkspace {
-    size_t cnt_off, masks_off, bitvals_off, colm_off, colb_off, scores_off, part_off, total;
+    size_t cnt_off, masks_off, bitvals_off, colm_off, colb_off, colp_off, recs_off, scores_off, part_off, total;
 };
 
 // Row blocks of count_bits_kernel: about 1024 blocks in all (four per CU of a
@@ -584,8 +805,10 @@ PreWorkspace layout(int w, int64_t V, int nt)
     L.masks_off = al(L.cnt_off + sizeof(int32_t) * (size_t)V * 8 * w);
     L.bitvals_off = al(L.masks_off + (size_t)w * V * nt);
     L.colm_off = al(L.bitvals_off + (size_t)w * V * nt);
-    L.colb_off = al(L.colm_off + (size_t)w * V * kMaxThresholds);
-    L.scores_off = al(L.colb_off + (size_t)w * V * kMaxThresholds);
+    L.colb_off = al(L.colm_off + (size_t)w * (V + 1) * kMaxThresholds);
+    L.colp_off = al(L.colb_off + (size_t)w * (V + 1) * kMaxThresholds);
+    L.recs_off = al(L.colp_off + sizeof(uint32_t) * (size_t)(V + 1) * kMaxThresholds);
+    L.scores_off = al(L.recs_off + score_recs_bytes(w, V));
     L.part_off = al(L.scores_off + sizeof(unsigned long long) * kMaxThresholds);
     L.total = al(L.part_off + sizeof(int32_t) * (size_t)max_count_row_blocks(V) * V * 8 * w);
     return L;
@@ -602,6 +825,7 @@ int preprocess_typed(const void *data, int64_t N, int64_t V, const Thresholds &t
     T *bitvals = (T *)(ws + L.bitvals_off);
     T *colm = (T *)(ws + L.colm_off);
     T *colb = (T *)(ws + L.colb_off);
+    uint32_t *colp = (uint32_t *)(ws + L.colp_off);
     unsigned long long *scores = (unsigned long long *)(ws + L.scores_off);
     int32_t *part = (int32_t *)(ws + L.part_off);
     const int64_t ncnt = V * 8 * (int64_t)sizeof(T);
@@ -621,7 +845,7 @@ int preprocess_typed(const void *data, int64_t N, int64_t V, const Thresholds &t
     IBP_LAUNCH_CHECK();
     const int64_t mthreads = V * kMaxThresholds;
     hipLaunchKernelGGL((create_masks_kernel<T>), dim3((unsigned)((mthreads + 255) / 256)), dim3(256), 0, stream, cnt, N,
-                       V, th, masks, bitvals, colm, colb);
+                       V, th, masks, bitvals, colm, colb, colp);
     IBP_LAUNCH_CHECK();
     // Rows of whole 16-byte chunks of 4- or 8-byte elements: the row-per-lane
     // kernel (about 5x fewer vector instructions per row than lane-per-column
@@ -635,30 +859,12 @@ int preprocess_typed(const void *data, int64_t N, int64_t V, const Thresholds &t
         const bool rows_path = (V * (int64_t)sizeof(T)) % 16 == 0 && V <= (int64_t)1 << 20 &&
                                ((uintptr_t)data % 16) == 0 &&
                                (th.count > 1 || N >= (int64_t)dev.num_cus * 4 * kWave);
-        // Rows of up to 512 bytes whose stride spreads over the LDS banks: a
-        // wave per 64-row tile copied to LDS (25.6 KB at 400-byte rows: six
-        // such waves per CU).
+        // Rows of up to 512 bytes whose stride spreads over the LDS banks:
+        // 64-row tiles copied to LDS (25.6 KB at 400-byte rows).
         const int64_t row_bytes = V * (int64_t)sizeof(T);
-        if (!scored && rows_path && tile_rows_ok(row_bytes)) {
-            const size_t lds = (size_t)kWave * (size_t)row_bytes;
-            const int64_t ntiles = (N + kWave - 1) / kWave;
-            int64_t per_cu = (int64_t)(160 * 1024) / (int64_t)lds;
-            if (per_cu > 8) per_cu = 8;
-            if (per_cu < 1) per_cu = 1;
-            int64_t grid = (int64_t)dev.num_cus * per_cu;
-            if (grid > ntiles) grid = ntiles;
-            const bool nested = thresholds_nested(th);
-            if (th.count <= 1)
-                hipLaunchKernelGGL((score_tile_kernel<T, 1, true>), dim3((unsigned)grid), dim3(kWave), lds, stream,
-                                   (const T *)data, N, (int)V, (const T *)colm, (const T *)colb, th.count, scores);
-            else if (nested)
-                hipLaunchKernelGGL((score_tile_kernel<T, kMaxThresholds, true>), dim3((unsigned)grid), dim3(kWave), lds,
-                                   stream, (const T *)data, N, (int)V, (const T *)colm, (const T *)colb, th.count,
-                                   scores);
-            else
-                hipLaunchKernelGGL((score_tile_kernel<T, kMaxThresholds, false>), dim3((unsigned)grid), dim3(kWave),
-                                   lds, stream, (const T *)data, N, (int)V, (const T *)colm, (const T *)colb, th.count,
-                                   scores);
+        if (!scored && rows_path && tile_rows_ok(row_bytes) && th.count <= 7) {
+            launch_score_tile<T>((const T *)data, N, V, (const T *)colm, (const T *)colb, colp, th.count,
+                                 thresholds_nested(th), scores, ws + L.recs_off, dev.num_cus, stream);
             IBP_LAUNCH_CHECK();
             scored = true;
         }
