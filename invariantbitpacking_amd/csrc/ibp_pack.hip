@@ -21,6 +21,7 @@
 #include "ibp_bits.h"
 #include "ibp_fetch_dev.h"
 #include "ibp_internal.h"
+#include "ibp_tile.h"
 #include "ibp_wave.h"
 
 namespace ibp {
@@ -118,6 +119,102 @@ __global__ __launch_bounds__(256) void size_kernel(const T *data, int64_t num_ve
     if (lane == 0) {
         if (total && acc) atomicAdd(total, acc);
         if (err && bad) atomicAdd(err, bad);
+    }
+}
+
+// get_compress_size of a whole table (no index, one mask) in 64-row tiles
+// copied to LDS: lane r scores row r of the tile.  A row saves the mask
+// popcounts of its flagged elements, i.e. the sum of all popcounts minus
+// those of the elements that are not flagged (lost_step).  Mask and bitval
+// are uniform scalar loads straight from the arguments, a chunk ahead (the
+// last chunk's look-ahead re-reads it), popcounts on the scalar unit.
+template <typename T>
+__device__ __forceinline__ void size_score_chunk(const uint4 &v, const T (&m)[16 / sizeof(T)],
+                                                 const T (&b)[16 / sizeof(T)], uint32_t &lost)
+{
+#pragma unroll
+    for (int e = 0; e < (int)(16 / sizeof(T)); ++e)
+        lost = lost_step<true>(xor_ordered(chunk_elem<T>(v, e), b[e]), m[e], (uint32_t)popc(m[e]), lost);
+}
+
+template <typename T>
+__device__ __forceinline__ void size_load_chunk(const T *__restrict__ mask, const T *__restrict__ bitval, int c,
+                                                T (&m)[16 / sizeof(T)], T (&b)[16 / sizeof(T)])
+{
+    constexpr int EPC = 16 / (int)sizeof(T);
+#pragma unroll
+    for (int e = 0; e < EPC; ++e) {
+        m[e] = mask[c * EPC + e];
+        b[e] = bitval[c * EPC + e];
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void size_need_chunk(const T (&m)[16 / sizeof(T)], const T (&b)[16 / sizeof(T)],
+                                                const uint4 &v)
+{
+#pragma unroll
+    for (int e = 0; e < (int)(16 / sizeof(T)); ++e) asm volatile("" ::"s"(m[e]), "s"(b[e]));
+    asm volatile("" ::"v"(v.x), "v"(v.y), "v"(v.z), "v"(v.w));
+}
+
+template <typename T>
+__global__ __launch_bounds__(kWave) void size_tile_kernel(const T *__restrict__ data, int64_t num_vecs, int V,
+                                                          const T *__restrict__ mask, const T *__restrict__ bitval,
+                                                          int64_t *__restrict__ sizes, unsigned long long *total)
+{
+    constexpr int W = (int)sizeof(T);
+    constexpr int EPC = 16 / W;
+    extern __shared__ __align__(16) unsigned char tile[];
+    const int lane = lane_id();
+    const int C = V * W / 16;
+    const int64_t ntiles = (num_vecs + kWave - 1) / kWave;
+    int pcs = 0;
+    for (int j = lane; j < V; j += kWave) pcs += popc(mask[j]);
+    const uint32_t pcsum = (uint32_t)wave_sum(pcs);
+    const uint4 *rowq = reinterpret_cast<const uint4 *>(tile + (size_t)lane * V * W);
+    unsigned long long acc = 0;
+    for (int64_t tl = blockIdx.x; tl < ntiles; tl += gridDim.x) {
+        copy_tile<T, 1>(data, num_vecs, V, C, tl, 0, lane, tile);
+        __builtin_amdgcn_s_waitcnt(0x0f70); // vmcnt(0): the tile has landed
+        __builtin_amdgcn_wave_barrier();
+        uint32_t lost = 0;
+        T ma[EPC], ba[EPC], mb[EPC], bb[EPC];
+        size_load_chunk<T>(mask, bitval, 0, ma, ba);
+        uint4 va = rowq[0], vb;
+        int c = 0;
+        for (; c + 1 < C; c += 2) {
+            size_need_chunk<T>(ma, ba, va);
+            size_load_chunk<T>(mask, bitval, c + 1, mb, bb);
+            vb = rowq[c + 1];
+            size_score_chunk<T>(va, ma, ba, lost);
+            size_need_chunk<T>(mb, bb, vb);
+            const int cn = c + 2 < C ? c + 2 : c + 1;
+            size_load_chunk<T>(mask, bitval, cn, ma, ba);
+            va = rowq[cn];
+            size_score_chunk<T>(vb, mb, bb, lost);
+        }
+        if (c < C) { // odd C: the last chunk
+            size_need_chunk<T>(ma, ba, va);
+            size_score_chunk<T>(va, ma, ba, lost);
+        }
+        const int64_t r = tl * kWave + lane;
+        if (r < num_vecs) {
+            const int64_t sz = slot_size(V, W, (int64_t)(pcsum - lost));
+            sizes[r] = sz;
+            acc += (unsigned long long)sz;
+        }
+        // the tile's LDS reads are done before the next copy lands
+        __builtin_amdgcn_s_waitcnt(0xc07f); // lgkmcnt(0)
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (total) {
+        // one atomic per wave: the lanes' sums as 24-bit low parts and the
+        // rest (each of 64 fits 32 bits while a lane's sum is below 2^49)
+        const uint32_t lo = (uint32_t)wave_sum((int)(uint32_t)(acc & 0xffffffull));
+        const uint32_t hi = (uint32_t)wave_sum((int)(uint32_t)(acc >> 24));
+        const unsigned long long tot = (unsigned long long)lo + ((unsigned long long)hi << 24);
+        if (lane == 0 && tot) atomicAdd(total, tot);
     }
 }
 
@@ -497,6 +594,293 @@ __global__ __launch_bounds__(256) void pack_small_kernel(PackParams p)
     if (p.err && bad && lane == 0) atomicAdd(p.err, bad);
 }
 
+// ---- Row-per-lane pack of 64-row tiles, in place, one mask.
+//
+// The tile is copied to LDS (global_load_lds), lane r packs row r there in
+// place, and the packed slots go back to the table with coalesced 16-byte
+// stores.  Per element a lane tests the flag, gathers the free bits of a
+// flagged element with the column's run table, and appends them to a word
+// accumulator; a completed word is written to the row in LDS.  Column
+// constants are uniform scalar loads, a chunk ahead.  In place is safe: after
+// element j at most j + 1 payload words are complete, so the words written
+// while chunk c is packed end inside chunk c + 1 (the header is at most one
+// chunk), which was read before chunk c's first write.
+//
+// Compared with a wave per row (pack_small_kernel: a wave scan, a ballot and
+// LDS atomics per row, about 78 vector and 76 scalar instructions per row on
+// the GNN table) a lane per row needs about 30 vector instructions per
+// element-lane, i.e. per row of 64 rows a wave packs together.
+
+// Column constants of the pack: mask and bitval, the runs of free bits of a
+// flagged element (run k moves up by sh[k] to its place in the MSB-first
+// payload), the payload bits of a flagged element, and the run count (more
+// than kRunCap: the generic gather).  Laid out [chunk][element], with a zero
+// chunk after the last.
+template <typename T>
+struct PackCol {
+    T m;
+    T b;
+    T run[kRunCap];
+    uint32_t sh[kRunCap];
+    uint32_t len;
+    uint32_t nruns;
+};
+
+// One block: thread j < (C + 1) * EPC builds column j's record; *max_runs is
+// the largest run count.
+template <typename T>
+__global__ __launch_bounds__(256) void pack_cols_kernel(const T *__restrict__ mask, const T *__restrict__ bitval,
+                                                        int64_t V, int ncols, PackCol<T> *cols, int *max_runs)
+{
+    constexpr int B = Bits<T>::B;
+    __shared__ int mx;
+    if (threadIdx.x == 0) mx = 0;
+    __syncthreads();
+    for (int j = threadIdx.x; j < ncols; j += blockDim.x) {
+        PackCol<T> c;
+        const T m = j < V ? mask[j] : T(0);
+        c.m = m;
+        c.b = j < V ? bitval[j] : T(0);
+        c.len = j < V ? (uint32_t)(B - popc(m)) : 0u;
+        for (int k = 0; k < kRunCap; ++k) {
+            c.run[k] = T(0);
+            c.sh[k] = 0;
+        }
+        T z = j < V ? T(~m) : T(0);
+        int filled = 0, runs = 0;
+        while (z) {
+            const int sft = clz(z);
+            const int r = clz(T(~T(z << sft)));
+            const T run = shr(himask<T>(r), sft);
+            if (runs < kRunCap) {
+                c.run[runs] = run;
+                c.sh[runs] = (uint32_t)(sft - filled);
+            }
+            filled += r;
+            z &= ~run;
+            ++runs;
+        }
+        c.nruns = (uint32_t)runs;
+        cols[j] = c;
+        atomicMax(&mx, runs);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *max_runs = mx;
+}
+
+// The constants of one chunk in SGPRs (R runs per element; R < 0: the generic
+// gather, which needs only the mask).
+template <typename T, int R>
+struct PackChunkCols {
+    static constexpr int EPC = 16 / (int)sizeof(T);
+    static constexpr int NR = R > 0 ? R : 1;
+    T m[EPC], b[EPC], run[EPC][NR];
+    uint32_t sh[EPC][NR], len[EPC];
+};
+
+template <typename T, int R>
+__device__ __forceinline__ void load_pack_cols(const PackCol<T> *__restrict__ c, PackChunkCols<T, R> &k)
+{
+    constexpr int EPC = PackChunkCols<T, R>::EPC;
+#pragma unroll
+    for (int e = 0; e < EPC; ++e) {
+        const T *f = reinterpret_cast<const T *>(c + e); // m, b, run[kRunCap]
+        const uint32_t *g = reinterpret_cast<const uint32_t *>(f + 2 + kRunCap); // sh[kRunCap], len
+        k.m[e] = f[0];
+        k.b[e] = f[1];
+#pragma unroll
+        for (int q = 0; q < (R > 0 ? R : 0); ++q) {
+            k.run[e][q] = f[2 + q];
+            k.sh[e][q] = g[q];
+        }
+        k.len[e] = g[kRunCap];
+    }
+}
+
+// Empty asm that needs the chunk's constants in SGPRs (places the wait for
+// their scalar loads).
+template <typename T, int R>
+__device__ __forceinline__ void need_pack_cols(const PackChunkCols<T, R> &k)
+{
+    constexpr int EPC = PackChunkCols<T, R>::EPC;
+#pragma unroll
+    for (int e = 0; e < EPC; ++e) {
+        asm volatile("" ::"s"(k.m[e]), "s"(k.b[e]), "s"(k.len[e]));
+#pragma unroll
+        for (int q = 0; q < (R > 0 ? R : 0); ++q) asm volatile("" ::"s"(k.run[e][q]), "s"(k.sh[e][q]));
+    }
+}
+
+// Per-lane state of one row being packed.
+template <typename T>
+struct RowPacker {
+    T acc;        // the payload word being filled, MSB first
+    int nb;       // bits already in acc (< B)
+    int wpos;     // slot word the next completed word goes to
+    uint64_t fl;  // flags, bit j for element j
+};
+
+// Packs the EPC elements of chunk c (16 bytes v) of a row: completed words go
+// to row[wpos], or to the lane's dummy word when none completes (no branch).
+template <typename T, int R>
+__device__ __forceinline__ void pack_chunk(const uint4 &v, const PackChunkCols<T, R> &k, int c, T *row, T *dummy,
+                                           RowPacker<T> &st)
+{
+    constexpr int B = Bits<T>::B;
+    constexpr int EPC = PackChunkCols<T, R>::EPC;
+#pragma unroll
+    for (int e = 0; e < EPC; ++e) {
+        const int j = c * EPC + e;
+        const T x = chunk_elem<T>(v, e);
+        const bool fs = (T)(x & k.m[e]) == k.b[e];
+        st.fl |= fs ? (uint64_t(1) << j) : uint64_t(0);
+        const int L = fs ? (int)k.len[e] : B;
+        T ext;
+        if constexpr (R < 0) {
+            ext = extract_msb<T>(x, T(~k.m[e]));
+        } else {
+            ext = T(0);
+#pragma unroll
+            for (int q = 0; q < R; ++q) ext |= T(T(x & k.run[e][q]) << k.sh[e][q]);
+        }
+        const T val = fs ? ext : x;
+        st.acc |= T(val >> st.nb);
+        const int t = st.nb + L;
+        const bool full = t >= B;
+        *(full ? row + st.wpos : dummy) = st.acc;
+        // val << (B - nb), which is 0 for nb = 0
+        const T rem = T(T(val << 1) << (B - 1 - st.nb));
+        st.acc = full ? rem : st.acc;
+        st.wpos += full ? 1 : 0;
+        st.nb = t & (B - 1);
+    }
+}
+
+struct PackTileParams {
+    int64_t num_vecs;
+    int V;
+    const int *max_runs;   // the largest run count of the column records
+    uint32_t *bitmask;
+};
+
+template <typename T, int R>
+__device__ __forceinline__ void pack_tile_body(const PackTileParams &p, T *__restrict__ data,
+                                               const PackCol<T> *__restrict__ cols, unsigned char *tile, int *rowlen,
+                                               T *dummies)
+{
+    constexpr int W = (int)sizeof(T);
+    constexpr int EPC = 16 / W;
+    const int lane = lane_id();
+    const int V = p.V;
+    const int C = V * W / 16;
+    const int H = (int)(header_bytes(V, W) / W);
+    const int64_t ntiles = (p.num_vecs + kWave - 1) / kWave;
+    const int64_t ngroups = (p.num_vecs + 31) / 32;
+    T *const row = reinterpret_cast<T *>(tile) + (size_t)lane * V;
+    const uint4 *rowq = reinterpret_cast<const uint4 *>(row);
+    T *const dummy = dummies + lane;
+    // write-back walk: chunk g = lane + 64 i of the tile is (g / C, g % C)
+    const int q64 = kWave / C, r64 = kWave % C;
+    const int wr0 = lane / C, wc0 = lane - (lane / C) * C;
+    for (int64_t tl = blockIdx.x; tl < ntiles; tl += gridDim.x) {
+        copy_tile<T, 1>(data, p.num_vecs, V, C, tl, 0, lane, tile);
+        __builtin_amdgcn_s_waitcnt(0x0f70); // vmcnt(0): the tile has landed
+        __builtin_amdgcn_wave_barrier();
+        const int64_t r = tl * kWave + lane;
+        RowPacker<T> st;
+        st.acc = T(0);
+        st.nb = 0;
+        st.wpos = H;
+        st.fl = 0;
+        PackChunkCols<T, R> ka, kb;
+        load_pack_cols<T, R>(cols, ka);
+        uint4 va = rowq[0], vb;
+        int c = 0;
+        for (; c + 1 < C; c += 2) {
+            need_pack_cols<T, R>(ka);
+            load_pack_cols<T, R>(cols + (size_t)(c + 1) * EPC, kb);
+            vb = rowq[c + 1];
+            pack_chunk<T, R>(va, ka, c, row, dummy, st);
+            need_pack_cols<T, R>(kb);
+            load_pack_cols<T, R>(cols + (size_t)(c + 2) * EPC, ka); // chunk C is the zero one
+            va = rowq[c + 2 < C ? c + 2 : c + 1];
+            pack_chunk<T, R>(vb, kb, c + 1, row, dummy, st);
+        }
+        if (c < C) { // odd C: the last chunk
+            need_pack_cols<T, R>(ka);
+            pack_chunk<T, R>(va, ka, c, row, dummy, st);
+        }
+        *(st.nb > 0 ? row + st.wpos : dummy) = st.acc; // the last, partial word
+        st.wpos += st.nb > 0 ? 1 : 0;
+        // header: flag j in word j / B, LSB first (at most two words)
+        row[0] = (T)st.fl;
+        if (H > 1) row[1] = (T)(st.fl >> 32);
+        const bool packed = r < p.num_vecs && st.wpos < V;
+        rowlen[lane] = packed ? st.wpos : 0;
+        const uint64_t pk = ballot(packed);
+        if (lane == 0) {
+            p.bitmask[2 * tl] = (uint32_t)pk;
+            if (2 * tl + 1 < ngroups) p.bitmask[2 * tl + 1] = (uint32_t)(pk >> 32);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // packed slots back to the table: whole slots, zero past the row's
+        // words; raw rows stay as they are
+        uint4 *out = reinterpret_cast<uint4 *>(data + tl * kWave * (int64_t)V);
+        const int64_t rows_left = p.num_vecs - tl * kWave;
+        const int nch = (int)((rows_left < kWave ? rows_left : kWave) * C);
+        int wr = wr0, wc = wc0;
+        for (int g = lane; g < nch; g += kWave) {
+            const int len = rowlen[wr];
+            if (len > 0) {
+                uint4 v = reinterpret_cast<const uint4 *>(tile)[g];
+                const int w0 = wc * EPC; // first word of the chunk
+                if constexpr (W == 8) {
+                    if (w0 >= len) v.x = v.y = 0u;
+                    if (w0 + 1 >= len) v.z = v.w = 0u;
+                } else {
+                    if (w0 >= len) v.x = 0u;
+                    if (w0 + 1 >= len) v.y = 0u;
+                    if (w0 + 2 >= len) v.z = 0u;
+                    if (w0 + 3 >= len) v.w = 0u;
+                }
+                out[g] = v;
+            }
+            wr += q64;
+            wc += r64;
+            if (wc >= C) {
+                wc -= C;
+                ++wr;
+            }
+        }
+        // every LDS read of this tile is done before the next copy lands
+        __builtin_amdgcn_s_waitcnt(0xc07f); // lgkmcnt(0)
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// data and cols are separate restrict arguments so that the stores to the
+// table cannot alias the column records, which then stay scalar loads.
+template <typename T>
+__global__ __launch_bounds__(kWave) void pack_tile_kernel(PackTileParams p, T *__restrict__ data,
+                                                          const PackCol<T> *__restrict__ cols)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const size_t tile_bytes = (size_t)kWave * p.V * sizeof(T);
+    int *rowlen = reinterpret_cast<int *>(lds_raw + tile_bytes);
+    T *dummies = reinterpret_cast<T *>(lds_raw + tile_bytes + kWave * sizeof(int));
+    const int R = __builtin_amdgcn_readfirstlane(*p.max_runs);
+    switch (R) {
+    case 0: pack_tile_body<T, 0>(p, data, cols, lds_raw, rowlen, dummies); break;
+    case 1: pack_tile_body<T, 1>(p, data, cols, lds_raw, rowlen, dummies); break;
+    case 2: pack_tile_body<T, 2>(p, data, cols, lds_raw, rowlen, dummies); break;
+    case 3: pack_tile_body<T, 3>(p, data, cols, lds_raw, rowlen, dummies); break;
+    case 4: pack_tile_body<T, 4>(p, data, cols, lds_raw, rowlen, dummies); break;
+    default: pack_tile_body<T, -1>(p, data, cols, lds_raw, rowlen, dummies); break;
+    }
+}
+
 // ---- inclusive scan of int64 (the thrust::inclusive_scan of compress_condensed,
 // src/compress.cu:284-285): tile sums, an exclusive scan of the tile sums in
 // one block, then each tile scanned with its carry-in.
@@ -573,6 +957,23 @@ int size_typed(const void *data, int64_t num_vecs, int64_t V, const void *mask, 
                uint64_t *total, hipStream_t stream)
 {
     const DeviceInfo &dev = device_info();
+    if constexpr (sizeof(T) >= 4) {
+        // a whole table with one mask, rows the LDS tiles take: a lane per row
+        if (!index && !clusters && n == num_vecs && n > 0 && tile_rows_ok(V * (int64_t)sizeof(T)) &&
+            ((uintptr_t)data % 16) == 0) {
+            const size_t lds = (size_t)kWave * V * sizeof(T);
+            int64_t per_cu = (int64_t)(160 * 1024) / (int64_t)lds;
+            if (per_cu > 8) per_cu = 8;
+            const int64_t ntiles = (n + kWave - 1) / kWave;
+            int64_t grid = (int64_t)dev.num_cus * per_cu;
+            if (grid > ntiles) grid = ntiles;
+            hipLaunchKernelGGL((size_tile_kernel<T>), dim3((unsigned)grid), dim3(kWave), lds, stream, (const T *)data,
+                               num_vecs, (int)V, (const T *)mask, (const T *)bitval, sizes,
+                               (unsigned long long *)total);
+            IBP_LAUNCH_CHECK();
+            return IBP_OK;
+        }
+    }
     // 16 blocks of four waves per CU (8: 13% slower on 32768 x 1024 words)
     int64_t grid = (n * kWave + 255) / 256;
     if (grid > (int64_t)dev.num_cus * 16) grid = (int64_t)dev.num_cus * 16;
@@ -663,6 +1064,46 @@ int64_t *noalloc_scratch()
     return g_scratch_ptr[dev];
 }
 
+// In-place pack of a table of rows of <= 64 elements in 64-row tiles (see
+// pack_tile_kernel); the column records live in a stream-ordered allocation.
+template <typename T>
+int pack_tile_launch(const PackParams &p, hipStream_t stream)
+{
+    const DeviceInfo &dev = device_info();
+    constexpr int W = (int)sizeof(T);
+    const int V = (int)p.V;
+    const int C = V * W / 16;
+    const int ncols = (C + 1) * (16 / W);
+    const size_t cols_bytes = ((size_t)ncols * sizeof(PackCol<T>) + 255) / 256 * 256;
+    void *buf = nullptr;
+    IBP_HIP(hipMallocAsync(&buf, cols_bytes + 256, stream));
+    PackCol<T> *cols = (PackCol<T> *)buf;
+    int *max_runs = (int *)((unsigned char *)buf + cols_bytes);
+    hipLaunchKernelGGL((pack_cols_kernel<T>), dim3(1), dim3(256), 0, stream, (const T *)p.mask, (const T *)p.bitval,
+                       p.V, ncols, cols, max_runs);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) {
+        PackTileParams tp;
+        tp.num_vecs = p.num_vecs;
+        tp.V = V;
+        tp.max_runs = max_runs;
+        tp.bitmask = p.bitmask;
+        const size_t lds = (size_t)kWave * V * W + kWave * sizeof(int) + kWave * W;
+        int64_t per_cu = (int64_t)(160 * 1024) / (int64_t)lds;
+        if (per_cu > 8) per_cu = 8;
+        if (per_cu < 1) per_cu = 1;
+        const int64_t ntiles = (p.num_vecs + kWave - 1) / kWave;
+        int64_t grid = (int64_t)dev.num_cus * per_cu;
+        if (grid > ntiles) grid = ntiles;
+        hipLaunchKernelGGL((pack_tile_kernel<T>), dim3((unsigned)grid), dim3(kWave), lds, stream, tp, (T *)p.out,
+                           (const PackCol<T> *)cols);
+        e = hipGetLastError();
+    }
+    IBP_HIP(hipFreeAsync(buf, stream));
+    if (e != hipSuccess) return set_error(IBP_ERR_HIP, "pack kernel launch failed: %s", hipGetErrorString(e));
+    return IBP_OK;
+}
+
 template <typename T, typename IdxT, bool CONDENSED>
 int pack_typed(PackParams p, hipStream_t stream)
 {
@@ -670,6 +1111,13 @@ int pack_typed(PackParams p, hipStream_t stream)
     const int64_t buf_bytes = slot_buf_bytes(p.V, (int)sizeof(T));
     const int64_t lds_budget = 64 * 1024;
     const int64_t ngroups = (p.n + 31) / 32;
+    if constexpr (sizeof(T) >= 4 && !CONDENSED) {
+        // in place, no index, one mask, rows the LDS tiles take: a lane per row
+        if (p.in == (const void *)p.out && p.index == nullptr && p.clusters == nullptr && p.bitmask != nullptr &&
+            p.n == p.num_vecs && p.n > 0 && p.V <= kWave && tile_rows_ok(p.V * (int64_t)sizeof(T)) &&
+            ((uintptr_t)p.in % 16) == 0)
+            return pack_tile_launch<T>(p, stream);
+    }
     if (ngroups < (int64_t)dev.num_cus * 8) {
         // Too few 32-row groups for a wave each: the waves of a block share a
         // group (16 per block with slot buffers in device memory, as many as

@@ -14,6 +14,7 @@
 
 #include "ibp_bits.h"
 #include "ibp_internal.h"
+#include "ibp_tile.h"
 #include "ibp_wave.h"
 
 namespace ibp {
@@ -252,59 +253,10 @@ __global__ __launch_bounds__(256) void score_kernel(const T *data, int64_t num_v
 // threshold, and for an ascending list of thresholds >= 0.5, where the masks
 // shrink as t grows), so one XOR per element serves all thresholds.
 template <typename T>
-__device__ __forceinline__ T chunk_elem(const uint4 &v, int e)
-{
-    if constexpr (sizeof(T) == 8) {
-        return e == 0 ? (T)(((uint64_t)v.y << 32) | v.x) : (T)(((uint64_t)v.w << 32) | v.z);
-    } else {
-        return (T)(e == 0 ? v.x : e == 1 ? v.y : e == 2 ? v.z : v.w);
-    }
-}
-
-template <typename T>
 __device__ __forceinline__ uint32_t any_bit(T u)
 {
     if constexpr (sizeof(T) == 8) return (uint32_t)u | (uint32_t)((uint64_t)u >> 32);
     else return (uint32_t)u;
-}
-
-// lost + (d & m != 0 ? popc(m) : 0) as AND, (AND-OR), MIN, MAD_U24 with the
-// mask and its popcount in SGPRs (one scalar operand per instruction).
-// ORDERED keeps the steps in program order with the other volatile asm of the
-// loop, so the scalar loads of the next column stay ahead of them and their
-// wait behind them.
-#define IBP_LOST_STEP64                                                                                                \
-    "v_and_b32 %0, %2, %3\n\t"                                                                                          \
-    "v_and_or_b32 %0, %4, %5, %0\n\t"                                                                                   \
-    "v_min_u32 %0, 1, %0\n\t"                                                                                           \
-    "v_mad_u32_u24 %1, %0, %6, %1"
-#define IBP_LOST_STEP32                                                                                                \
-    "v_and_b32 %0, %2, %3\n\t"                                                                                          \
-    "v_min_u32 %0, 1, %0\n\t"                                                                                           \
-    "v_mad_u32_u24 %1, %0, %4, %1"
-template <bool ORDERED = false>
-__device__ __forceinline__ uint32_t lost_step(uint64_t d, uint64_t m, uint32_t pc, uint32_t lost)
-{
-    uint32_t z;
-    if constexpr (ORDERED)
-        asm volatile(IBP_LOST_STEP64
-                     : "=&v"(z), "+v"(lost)
-                     : "s"((uint32_t)m), "v"((uint32_t)d), "v"((uint32_t)(d >> 32)), "s"((uint32_t)(m >> 32)), "s"(pc));
-    else
-        asm(IBP_LOST_STEP64
-            : "=&v"(z), "+v"(lost)
-            : "s"((uint32_t)m), "v"((uint32_t)d), "v"((uint32_t)(d >> 32)), "s"((uint32_t)(m >> 32)), "s"(pc));
-    return lost;
-}
-template <bool ORDERED = false>
-__device__ __forceinline__ uint32_t lost_step(uint32_t d, uint32_t m, uint32_t pc, uint32_t lost)
-{
-    uint32_t z;
-    if constexpr (ORDERED)
-        asm volatile(IBP_LOST_STEP32 : "=&v"(z), "+v"(lost) : "s"(m), "v"(d), "s"(pc));
-    else
-        asm(IBP_LOST_STEP32 : "=&v"(z), "+v"(lost) : "s"(m), "v"(d), "s"(pc));
-    return lost;
 }
 
 template <typename T, int NT, bool NESTED>
@@ -409,12 +361,6 @@ __global__ __launch_bounds__(256) void score_rows_kernel(const T *__restrict__ d
 // scoring the current one; that halves the blocks per CU and is slower.
 // Used where the row stride in dwords is not a multiple of 16 modulo 32, so
 // eight lanes reading 16 bytes each hit at most two bank groups twice.
-constexpr int kTileMaxRowBytes = 512;
-__host__ __device__ constexpr bool tile_rows_ok(int64_t row_bytes)
-{
-    return row_bytes % 16 == 0 && row_bytes <= kTileMaxRowBytes && (row_bytes / 4) % 32 != 0 &&
-           (row_bytes / 4) % 32 != 16;
-}
 
 // The column constants one wave of score_tile_kernel needs for one 16-byte
 // chunk of a row (EPC elements): masks and their popcounts for its TPW
@@ -482,22 +428,6 @@ __device__ __forceinline__ void load_rec(const ScoreRec<T, TPW, NB> *__restrict_
         for (int t = 0; t < NB; ++t) b[e][t] = rb[e * NB + t];
 }
 
-// x ^ b (b uniform) as volatile asm, so it stays in order with the steps.
-__device__ __forceinline__ uint64_t xor_ordered(uint64_t x, uint64_t b)
-{
-    uint32_t lo, hi;
-    asm volatile("v_xor_b32 %0, %2, %3\n\tv_xor_b32 %1, %4, %5"
-                 : "=&v"(lo), "=&v"(hi)
-                 : "s"((uint32_t)b), "v"((uint32_t)x), "s"((uint32_t)(b >> 32)), "v"((uint32_t)(x >> 32)));
-    return ((uint64_t)hi << 32) | lo;
-}
-__device__ __forceinline__ uint32_t xor_ordered(uint32_t x, uint32_t b)
-{
-    uint32_t d;
-    asm volatile("v_xor_b32 %0, %1, %2" : "=v"(d) : "s"(b), "v"(x));
-    return d;
-}
-
 // Empty asm that needs the record's values in SGPRs: the wait for their
 // scalar loads is placed here.
 template <typename T, int TPW, int NB>
@@ -512,23 +442,6 @@ __device__ __forceinline__ void need_rec(const T (&m)[ScoreRec<T, TPW, NB>::EPC]
         for (int t = 0; t < TPW; ++t) asm volatile("" ::"s"(m[e][t]), "s"(p[e][t]));
 #pragma unroll
         for (int t = 0; t < NB; ++t) asm volatile("" ::"s"(b[e][t]));
-    }
-}
-
-// The copy of tile tl into `buf` by wave `wib` of WPT: chunks wib, wib + WPT, ...
-template <typename T, int WPT>
-__device__ __forceinline__ void copy_tile(const T *data, int64_t num_vecs, int V, int C, int64_t tl, int wib, int lane,
-                                          unsigned char *buf)
-{
-    const uint4 *src = reinterpret_cast<const uint4 *>(data + tl * kWave * (int64_t)V);
-    const int64_t rows_left = num_vecs - tl * kWave;
-    // the last tile may be short: its missing rows re-read the last chunk
-    const int64_t last = (rows_left < kWave ? rows_left : kWave) * C - 1;
-    for (int k = wib; k < C; k += WPT) {
-        const int64_t g = (int64_t)k * kWave + lane;
-        __builtin_amdgcn_global_load_lds(static_cast<const void *>(src + (g < last ? g : last)),
-                                         (__attribute__((address_space(3))) void *)(buf + (size_t)k * (kWave * 16)),
-                                         16, 0, 0);
     }
 }
 

@@ -1,0 +1,107 @@
+// ibp_tile.h -- 64-row tiles of a row-major table copied to LDS, shared by
+// the row-per-lane kernels (preprocess scoring, pack).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ibp_wave.h"
+
+namespace ibp {
+
+// Rows of up to 512 bytes, whole 16-byte chunks, whose stride in dwords is
+// not a multiple of 16 modulo 32: lane r reading 16 bytes of row r then
+// spreads eight lanes over the LDS banks with at most two hits per bank group.
+constexpr int kTileMaxRowBytes = 512;
+__host__ __device__ constexpr bool tile_rows_ok(int64_t row_bytes)
+{
+    return row_bytes % 16 == 0 && row_bytes <= kTileMaxRowBytes && (row_bytes / 4) % 32 != 0 &&
+           (row_bytes / 4) % 32 != 16;
+}
+
+// The copy of tile tl (rows tl*64 ...) into `buf` by wave `wib` of WPT: 16-byte
+// chunks wib, wib + WPT, ... of the tile, each instruction 1 KB of consecutive
+// bytes straight to LDS (global_load_lds_dwordx4: no registers, tracked by
+// vmcnt).  C = row bytes / 16.
+template <typename T, int WPT>
+__device__ __forceinline__ void copy_tile(const T *data, int64_t num_vecs, int V, int C, int64_t tl, int wib, int lane,
+                                          unsigned char *buf)
+{
+    const uint4 *src = reinterpret_cast<const uint4 *>(data + tl * kWave * (int64_t)V);
+    const int64_t rows_left = num_vecs - tl * kWave;
+    // the last tile may be short: its missing rows re-read the last chunk
+    const int64_t last = (rows_left < kWave ? rows_left : kWave) * C - 1;
+    for (int k = wib; k < C; k += WPT) {
+        const int64_t g = (int64_t)k * kWave + lane;
+        __builtin_amdgcn_global_load_lds(static_cast<const void *>(src + (g < last ? g : last)),
+                                         (__attribute__((address_space(3))) void *)(buf + (size_t)k * (kWave * 16)),
+                                         16, 0, 0);
+    }
+}
+
+// Element e of a 16-byte chunk of 4- or 8-byte words.
+template <typename T>
+__device__ __forceinline__ T chunk_elem(const uint4 &v, int e)
+{
+    if constexpr (sizeof(T) == 8) {
+        return e == 0 ? (T)(((uint64_t)v.y << 32) | v.x) : (T)(((uint64_t)v.w << 32) | v.z);
+    } else {
+        return (T)(e == 0 ? v.x : e == 1 ? v.y : e == 2 ? v.z : v.w);
+    }
+}
+
+// lost + (d & m != 0 ? popc(m) : 0) as AND, (AND-OR), MIN, MAD_U24 with the
+// mask and its popcount in SGPRs (one scalar operand per instruction).
+// ORDERED keeps the steps in program order with the other volatile asm of the
+// loop, so the scalar loads of the next column stay ahead of them and their
+// wait behind them.
+#define IBP_LOST_STEP64                                                                                                \
+    "v_and_b32 %0, %2, %3\n\t"                                                                                          \
+    "v_and_or_b32 %0, %4, %5, %0\n\t"                                                                                   \
+    "v_min_u32 %0, 1, %0\n\t"                                                                                           \
+    "v_mad_u32_u24 %1, %0, %6, %1"
+#define IBP_LOST_STEP32                                                                                                \
+    "v_and_b32 %0, %2, %3\n\t"                                                                                          \
+    "v_min_u32 %0, 1, %0\n\t"                                                                                           \
+    "v_mad_u32_u24 %1, %0, %4, %1"
+template <bool ORDERED = false>
+__device__ __forceinline__ uint32_t lost_step(uint64_t d, uint64_t m, uint32_t pc, uint32_t lost)
+{
+    uint32_t z;
+    if constexpr (ORDERED)
+        asm volatile(IBP_LOST_STEP64
+                     : "=&v"(z), "+v"(lost)
+                     : "s"((uint32_t)m), "v"((uint32_t)d), "v"((uint32_t)(d >> 32)), "s"((uint32_t)(m >> 32)), "s"(pc));
+    else
+        asm(IBP_LOST_STEP64
+            : "=&v"(z), "+v"(lost)
+            : "s"((uint32_t)m), "v"((uint32_t)d), "v"((uint32_t)(d >> 32)), "s"((uint32_t)(m >> 32)), "s"(pc));
+    return lost;
+}
+template <bool ORDERED = false>
+__device__ __forceinline__ uint32_t lost_step(uint32_t d, uint32_t m, uint32_t pc, uint32_t lost)
+{
+    uint32_t z;
+    if constexpr (ORDERED)
+        asm volatile(IBP_LOST_STEP32 : "=&v"(z), "+v"(lost) : "s"(m), "v"(d), "s"(pc));
+    else
+        asm(IBP_LOST_STEP32 : "=&v"(z), "+v"(lost) : "s"(m), "v"(d), "s"(pc));
+    return lost;
+}
+
+// x ^ b (b uniform) as volatile asm, so it stays in order with the steps.
+__device__ __forceinline__ uint64_t xor_ordered(uint64_t x, uint64_t b)
+{
+    uint32_t lo, hi;
+    asm volatile("v_xor_b32 %0, %2, %3\n\tv_xor_b32 %1, %4, %5"
+                 : "=&v"(lo), "=&v"(hi)
+                 : "s"((uint32_t)b), "v"((uint32_t)x), "s"((uint32_t)(b >> 32)), "v"((uint32_t)(x >> 32)));
+    return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ uint32_t xor_ordered(uint32_t x, uint32_t b)
+{
+    uint32_t d;
+    asm volatile("v_xor_b32 %0, %1, %2" : "=v"(d) : "s"(b), "v"(x));
+    return d;
+}
+
+} // namespace ibp

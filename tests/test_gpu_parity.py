@@ -137,6 +137,76 @@ def test_size_and_compress_match_oracle(ibp, oracle, w, shape):
         assert np.array_equal(_u(_np(t)), _u(ocomp)), "slots"
 
 
+def masked_rows(n, v, w, seed, kind, flag_frac=0.8):
+    """Rows drawn against a chosen mask/bitval so that about flag_frac of the
+    elements are flagged.  kind picks the free bits of each column: 'runsR'
+    (one to R runs, R <= 4), 'many' (alternating bits: more runs
+    than a run table holds), 'none' (mask all ones: a flagged element carries
+    no bits), 'mixed' (columns of the last three) or 'raw' (nothing flags)."""
+    rng = np.random.default_rng(seed)
+    B = 8 * w
+    full = (1 << B) - 1
+    masks = []
+    for j in range(v):
+        k = kind if kind != "mixed" else ("runs4", "many", "none")[j % 3]
+        if k.startswith("runs"):
+            # 1 + j % R runs (R = 1..4) in disjoint quarters of the word
+            m = full
+            for q in range(1 + j % int(k[4])):
+                lo = int(rng.integers(1, B // 8 + 1))
+                m &= ~(((1 << lo) - 1) << (q * (B // 4) + int(rng.integers(0, B // 4 - lo + 1))))
+        elif k == "many":
+            m = int(("01" * (B // 2))[: B], 2) if j % 2 else int(("0011" * (B // 4))[: B], 2)
+        elif k == "none":
+            m = full
+        else:
+            m = full
+        masks.append(m)
+    m = np.array(masks, dtype=np.uint64)
+    b = np.array([int(rng.integers(0, 1 << 62)) for _ in range(v)], dtype=np.uint64) & m
+    x = rng.integers(0, 1 << 63, size=(n, v), dtype=np.uint64) ^ rng.integers(0, 2, size=(n, v), dtype=np.uint64)
+    if kind != "raw":
+        flag = rng.random((n, v)) < flag_frac
+        x = np.where(flag, (x & ~m) | b, x)
+    dt = np.dtype("u%d" % w)
+    return (x.astype(dt).view(np.dtype("i%d" % w)), m.astype(dt), b.astype(dt))
+
+
+# Rows of whole 16-byte chunks whose stride spreads over the LDS banks take
+# the row-per-lane tile size and pack kernels; the shapes cover one and an odd
+# number of chunks, one- and two-word headers and short last tiles.
+TILE_SHAPES = [(500, 50, 8), (300, 2, 8), (1000, 62, 8), (777, 12, 4), (2000, 60, 4), (130, 36, 4), (64, 4, 4),
+               (70000, 50, 8)]
+
+
+@pytest.mark.parametrize("shape", TILE_SHAPES)
+@pytest.mark.parametrize("kind", ["float", "runs1", "runs2", "runs3", "runs4", "many", "none", "mixed", "raw"])
+def test_tile_size_and_pack_match_oracle(ibp, oracle, shape, kind):
+    n, v, w = shape
+    if kind == "float":
+        data = float_rows(n, v, w, seed=n + v + w)
+        om, ob, _, _ = oracle.preprocess(data, None)
+    else:
+        data, om, ob = masked_rows(n, v, w, seed=n + 3 * v + w, kind=kind)
+    mask = torch.from_numpy(om.view(np.dtype("i%d" % w))).cuda()
+    bitval = torch.from_numpy(ob.view(np.dtype("i%d" % w))).cuda()
+    ocomp, obm = oracle.compress(data, om, ob)
+    osz, _ = oracle.get_compress_size(data, om, ob)
+    print("tile pack", shape, kind, flush=True)
+    t = place(data, "device")
+    ctr = torch.tensor([5], dtype=torch.int64, device="cuda")
+    sz = ibp.get_compress_size(t, mask, bitval, None, ctr)
+    torch.cuda.synchronize()
+    assert np.array_equal(_np(sz), osz), "sizes"
+    assert int(ctr[0]) == 5 + int(osz.sum())
+    bm = ibp.compress_inplace(t, mask, bitval)
+    torch.cuda.synchronize()
+    assert np.array_equal(_np(bm), obm), "bitmask"
+    assert np.array_equal(_u(_np(t)), _u(ocomp)), "slots"
+    out = ibp.decompress_fetch(t, mask, bitval, bm, torch.device("cuda"))
+    assert np.array_equal(_np(out), data)
+
+
 def test_golden_slots_on_gpu(ibp):
     with open(GOLDEN) as f:
         cases = json.load(f)["slots"]
