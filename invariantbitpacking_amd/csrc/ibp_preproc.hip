@@ -32,18 +32,17 @@ using WideT = typename std::conditional<sizeof(T) == 8, uint64_t, uint32_t>::typ
 // Column-bit counts of the row block blockIdx.x and the 64-column tile
 // blockIdx.y: part[blockIdx.x][k*V + j] = rows of the block whose column j has
 // bit k set.  Lane <-> column; the block's waves split the rows; every thread
-// keeps its B counters in registers.  Rows are taken 15 at a time (15 loads in
-// flight) and added into bit-sliced 4-bit counters with a carry-save adder --
-// 7 logic operations per row instead of B extract-and-adds -- which are folded
-// into the B counters once per 15 rows.  The waves' counters are summed in LDS
-// and stored once per block (no global atomics: every block writes its own
-// partial counts, reduce_counts_kernel adds them up).
+// keeps its B counters in registers.  Rows are taken 16 at a time (16 loads in
+// flight) and added into bit-sliced counters (VCounter), folded into the B
+// counters every 240 rows.  The waves' counters are summed in LDS and stored
+// once per block (no global atomics: every block writes its own partial
+// counts, reduce_counts_kernel adds them up).
 template <typename T>
 __global__ __launch_bounds__(256) void count_bits_kernel(const T *data, int64_t num_vecs, int64_t V,
                                                          int64_t rows_per_block, int32_t *part)
 {
     constexpr int B = 8 * (int)sizeof(T);
-    constexpr int CHUNK = 15; // 4-bit counters
+    constexpr int CHUNK = 16;
     using W = WideT<T>;
     __shared__ int32_t sum[B * kWave];
     const int lane = lane_id();
@@ -58,6 +57,7 @@ __global__ __launch_bounds__(256) void count_bits_kernel(const T *data, int64_t 
     int32_t c[B];
 #pragma unroll
     for (int k = 0; k < B; ++k) c[k] = 0;
+    VCounter<W, B> vc;
     for (int64_t r = r0 + wave_in_block; r < r1; r += (int64_t)CHUNK * waves) {
         W x[CHUNK];
 #pragma unroll
@@ -65,8 +65,9 @@ __global__ __launch_bounds__(256) void count_bits_kernel(const T *data, int64_t 
             const int64_t rr = r + (int64_t)q * waves;
             x[q] = (col && rr < r1) ? (W)data[rr * V + j] : W(0);
         }
-        csa_count<W, B, CHUNK>(x, c);
+        vc.add16(x, c);
     }
+    vc.flush(c);
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < B; ++k)
