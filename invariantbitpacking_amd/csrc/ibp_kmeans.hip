@@ -43,38 +43,43 @@ __global__ void km_gather_kernel(const T *data, int64_t V, const int64_t *rows, 
 }
 
 // Nearest centroid by popc(x ^ c); equal distances go to the highest index.
-// One wave per group of kAssignRows rows; each centroid word is loaded once
-// per group (the centroids are re-read from L2 by every group, so more rows
-// per group means less L2 traffic) and compared with the group's elements,
-// 16 centroids at a time (16 x kAssignRows distance accumulators per lane);
-// each distance is then summed over the wave.
+// Blocks of four waves.  SPLIT = 1: a wave per group of kAssignRows rows.
+// SPLIT = 4 (rows of at least 256 elements): the four waves share one group
+// and wave w takes the columns j = lane + 64 (w + 4k), which gives a few
+// thousand long rows four times the waves (their assignment is
+// latency-bound).  Each centroid word is loaded once per group and compared
+// with the group's elements, 16 centroids at a time: 16 x kAssignRows = 64
+// distance accumulators per lane, summed over the wave in one exchange-and-add
+// pass (wave_sum64: lane l gets (centroid l / 4, row l % 4)).  With SPLIT = 4
+// the four waves' partial sums meet in LDS.  The nearest of the 16 centroids
+// for each row is then a lane minimum of (distance, 15 - centroid) over the
+// 16 lanes of that row, so lane l tracks row l % 4's best so far.
 constexpr int kAssignGroup = 16;
 constexpr int kAssignRows = 4;
-template <typename T>
-__global__ __launch_bounds__(256) void km_assign_kernel(const T *data, int64_t N, int64_t V, const T *centroids,
-                                                        int K, int32_t *cluster)
+constexpr int kAssignWaves = 4;
+static_assert(kAssignGroup * kAssignRows == 64, "one distance per lane after the wave sum");
+template <typename T, int SPLIT>
+__global__ __launch_bounds__(kWave * kAssignWaves) void km_assign_kernel(const T *data, int64_t N, int64_t V,
+                                                                         const T *centroids, int K, int32_t *cluster)
 {
+    constexpr int GPB = kAssignWaves / SPLIT; // row groups per block
+    __shared__ int part[SPLIT][kWave];
     const int lane = lane_id();
-    const int64_t wave = global_wave_id();
-    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
-    for (int64_t i0 = wave * kAssignRows; i0 < N; i0 += nwaves * kAssignRows) {
-        int best[kAssignRows], best_d[kAssignRows];
-#pragma unroll
-        for (int r = 0; r < kAssignRows; ++r) {
-            best[r] = 0;
-            best_d[r] = 0x7fffffff;
-        }
+    const int wv = wave_in_block_id();
+    const int sl = wv % SPLIT; // this wave's column slice
+    const int rr = lane % kAssignRows, gl = lane / kAssignRows; // the (row, centroid) lane l ends with
+    for (int64_t i0 = ((int64_t)blockIdx.x * GPB + wv / SPLIT) * kAssignRows; i0 < N;
+         i0 += (int64_t)gridDim.x * GPB * kAssignRows) {
+        int best = 0, best_d = 0x7fffffff; // of row rr
         // rows past N repeat row i0 (valid memory); their result is dropped
         int64_t row[kAssignRows];
 #pragma unroll
         for (int r = 0; r < kAssignRows; ++r) row[r] = i0 + r < N ? i0 + r : i0;
         for (int c0 = 0; c0 < K; c0 += kAssignGroup) {
-            int d[kAssignGroup][kAssignRows];
+            int d[kAssignGroup * kAssignRows];
 #pragma unroll
-            for (int g = 0; g < kAssignGroup; ++g)
-#pragma unroll
-                for (int r = 0; r < kAssignRows; ++r) d[g][r] = 0;
-            for (int64_t j = lane; j < V; j += kWave) {
+            for (int q = 0; q < kAssignGroup * kAssignRows; ++q) d[q] = 0;
+            for (int64_t j = (int64_t)sl * kWave + lane; j < V; j += kWave * SPLIT) {
                 KW<T> x[kAssignRows];
 #pragma unroll
                 for (int r = 0; r < kAssignRows; ++r) x[r] = (KW<T>)data[row[r] * V + j];
@@ -83,26 +88,33 @@ __global__ __launch_bounds__(256) void km_assign_kernel(const T *data, int64_t N
                     if (c0 + g >= K) break; // uniform
                     const KW<T> cw = (KW<T>)centroids[(int64_t)(c0 + g) * V + j];
 #pragma unroll
-                    for (int r = 0; r < kAssignRows; ++r) d[g][r] += popc((KW<T>)(x[r] ^ cw));
+                    for (int r = 0; r < kAssignRows; ++r) d[g * kAssignRows + r] += popc((KW<T>)(x[r] ^ cw));
                 }
             }
+            int dist = wave_sum64(d);
+            if constexpr (SPLIT > 1) {
+                part[sl][lane] = dist;
+                __syncthreads();
+                dist = 0;
 #pragma unroll
-            for (int g = 0; g < kAssignGroup; ++g) {
-                if (c0 + g >= K) break; // uniform
+                for (int w = 0; w < SPLIT; ++w) dist += part[w][lane];
+                __syncthreads(); // the partials are read before the next group's
+            }
+            // nearest of the group for row rr: minimum of (dist, 15 - g)
+            // over the 16 lanes of the row (lane bits 2..5)
+            int key = c0 + gl < K ? (dist << 4) | (kAssignGroup - 1 - gl) : 0x7fffffff;
 #pragma unroll
-                for (int r = 0; r < kAssignRows; ++r) {
-                    const int dist = wave_sum(d[g][r]);
-                    if (dist <= best_d[r]) {
-                        best_d[r] = dist;
-                        best[r] = c0 + g;
-                    }
-                }
+            for (int m = kAssignRows; m < kWave; m <<= 1) {
+                const int o = (int)shfl((uint32_t)key, lane ^ m);
+                key = o < key ? o : key;
+            }
+            const int gd = key >> 4;
+            if (gd <= best_d) {
+                best_d = gd;
+                best = c0 + kAssignGroup - 1 - (key & 15);
             }
         }
-        if (lane == 0)
-#pragma unroll
-            for (int r = 0; r < kAssignRows; ++r)
-                if (i0 + r < N) cluster[i0 + r] = best[r];
+        if (sl == 0 && lane < kAssignRows && i0 + lane < N) cluster[i0 + lane] = best;
     }
 }
 
@@ -184,8 +196,8 @@ __global__ void km_scatter_kernel(const int32_t *cluster, int64_t N, const int32
 
 // bits_set[(c*V + j)*B + k] = members of cluster c whose column j has bit k set.
 // grid = (column tiles, K, row splits); lane <-> column, the block's waves
-// split the cluster's rows, 15 rows at a time (15 gathered loads in flight)
-// added with a carry-save adder (csa_count).  The waves' counters are summed
+// split the cluster's rows, 16 rows at a time (16 gathered loads in flight)
+// added with carry-save adders (VCounter).  The waves' counters are summed
 // in LDS; with one row split the block owns its counters and stores them,
 // otherwise it adds them to counters zeroed beforehand.
 template <typename T>
@@ -194,7 +206,7 @@ __global__ __launch_bounds__(256) void km_bitcount_kernel(const T *data, int64_t
                                                           int32_t *bits_set)
 {
     constexpr int B = 8 * (int)sizeof(T);
-    constexpr int CHUNK = 15;
+    constexpr int CHUNK = 16;
     using W = KW<T>;
     __shared__ int32_t sum[B * kWave];
     const int lane = lane_id();
@@ -210,6 +222,7 @@ __global__ __launch_bounds__(256) void km_bitcount_kernel(const T *data, int64_t
     int32_t acc[B];
 #pragma unroll
     for (int k = 0; k < B; ++k) acc[k] = 0;
+    VCounter<W, B> vc;
     for (int64_t q = (int64_t)blockIdx.z * waves + wv; q < nrows; q += (int64_t)CHUNK * stride) {
         W x[CHUNK];
 #pragma unroll
@@ -217,8 +230,9 @@ __global__ __launch_bounds__(256) void km_bitcount_kernel(const T *data, int64_t
             const int64_t qi = q + (int64_t)u * stride;
             x[u] = (col && qi < nrows) ? (W)data[(int64_t)rows[qi] * V + j] : W(0);
         }
-        csa_count<W, B, CHUNK>(x, acc);
+        vc.add16(x, acc);
     }
+    vc.flush(acc);
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < B; ++k)
@@ -249,7 +263,7 @@ __global__ __launch_bounds__(256) void km_bitcount_chunks_kernel(const T *data, 
                                                                  int32_t *part)
 {
     constexpr int B = 8 * (int)sizeof(T);
-    constexpr int CHUNK = 15;
+    constexpr int CHUNK = 16;
     using W = KW<T>;
     __shared__ int32_t sum[B * kWave];
     const int e = blockIdx.y;
@@ -264,6 +278,7 @@ __global__ __launch_bounds__(256) void km_bitcount_chunks_kernel(const T *data, 
     int32_t acc[B];
 #pragma unroll
     for (int k = 0; k < B; ++k) acc[k] = 0;
+    VCounter<W, B> vc;
     for (int64_t q = q0 + wv; q < q1; q += (int64_t)CHUNK * waves) {
         W x[CHUNK];
 #pragma unroll
@@ -271,8 +286,9 @@ __global__ __launch_bounds__(256) void km_bitcount_chunks_kernel(const T *data, 
             const int64_t qi = q + (int64_t)u * waves;
             x[u] = (col && qi < q1) ? (W)data[(int64_t)perm[qi] * V + j] : W(0);
         }
-        csa_count<W, B, CHUNK>(x, acc);
+        vc.add16(x, acc);
     }
+    vc.flush(acc);
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < B; ++k)
@@ -474,7 +490,9 @@ int kmeans_typed(const void *data_v, int64_t N, int64_t V, int K, int64_t seed, 
     hipLaunchKernelGGL(km_zero_kernel, dim3(blocks_for(N)), dim3(256), 0, s, cluster, N);
     IBP_LAUNCH_CHECK();
 
-    int64_t assign_grid = ((N + kAssignRows - 1) / kAssignRows * kWave + 255) / 256;
+    const bool assign_split = V >= kWave * kAssignWaves;
+    const int64_t groups = (N + kAssignRows - 1) / kAssignRows;
+    int64_t assign_grid = assign_split ? groups : (groups + kAssignWaves - 1) / kAssignWaves;
     if (assign_grid > (int64_t)dev.num_cus * 8) assign_grid = (int64_t)dev.num_cus * 8;
     if (assign_grid < 1) assign_grid = 1;
     const unsigned tiles = (unsigned)((V + kWave - 1) / kWave);
@@ -490,8 +508,12 @@ int kmeans_typed(const void *data_v, int64_t N, int64_t V, int K, int64_t seed, 
     int32_t *chunk_count = chunked ? chunk_base + K : nullptr;
     int32_t *nchunks = chunked ? chunk_count + K : nullptr;
     for (int it = 0; it < 10; ++it) {
-        hipLaunchKernelGGL((km_assign_kernel<T>), dim3((unsigned)assign_grid), dim3(256), 0, s, data, N, V, cent, K,
-                           cluster);
+        if (assign_split)
+            hipLaunchKernelGGL((km_assign_kernel<T, kAssignWaves>), dim3((unsigned)assign_grid),
+                               dim3(kWave * kAssignWaves), 0, s, data, N, V, cent, K, cluster);
+        else
+            hipLaunchKernelGGL((km_assign_kernel<T, 1>), dim3((unsigned)assign_grid), dim3(kWave * kAssignWaves), 0,
+                               s, data, N, V, cent, K, cluster);
         if (chunked) {
             hipLaunchKernelGGL(km_sort_kernel, dim3(1), dim3(1024), 0, s, cluster, N, K, count, offset, perm,
                                (int)chunk_rows, chunks, chunk_base, chunk_count, nchunks);

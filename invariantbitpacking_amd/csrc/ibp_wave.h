@@ -79,4 +79,26 @@ __device__ __forceinline__ uint64_t shfl(uint64_t x, int src)
     return ((uint64_t)hi << 32) | lo;
 }
 
+
+// Sums 64 values per lane over the wave: lane l returns the sum over all
+// lanes of v[l].  Six exchange-and-add halvings -- lane bit 5 keeps the upper
+// or lower 32 values and trades the other half with lane ^ 32, then bit 4
+// with lane ^ 16, ... -- so 63 exchanges and adds in all instead of 64 wave
+// reductions.  v is clobbered.
+__device__ __forceinline__ int wave_sum64(int (&v)[64])
+{
+    const int lane = lane_id();
+#pragma unroll
+    for (int st = 0; st < 6; ++st) {
+        const int m = 32 >> st, half = 32 >> st;
+        const bool hi = (lane & m) != 0;
+#pragma unroll
+        for (int i = 0; i < half; ++i) {
+            const int a = v[i], b = v[i + half];
+            const int send = hi ? a : b, keep = hi ? b : a;
+            v[i] = keep + (int)shfl((uint32_t)send, lane ^ m);
+        }
+    }
+    return v[0];
+}
 } // namespace ibp
