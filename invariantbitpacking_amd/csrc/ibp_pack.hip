@@ -173,25 +173,27 @@ __global__ __launch_bounds__(kWave) void size_tile_kernel(const T *__restrict__ 
     for (int j = lane; j < V; j += kWave) pcs += popc(mask[j]);
     const uint32_t pcsum = (uint32_t)wave_sum(pcs);
     const uint4 *rowq = reinterpret_cast<const uint4 *>(tile + (size_t)lane * V * W);
+    const bool swz = tile_swizzled((int64_t)V * W);
+    const int sw = tile_swz(lane, swz); // chunk c of this lane's row sits at c ^ sw
     unsigned long long acc = 0;
     for (int64_t tl = blockIdx.x; tl < ntiles; tl += gridDim.x) {
-        copy_tile<T, 1>(data, num_vecs, V, C, tl, 0, lane, tile);
+        copy_tile<T, 1>(data, num_vecs, V, C, tl, 0, lane, tile, swz);
         __builtin_amdgcn_s_waitcnt(0x0f70); // vmcnt(0): the tile has landed
         __builtin_amdgcn_wave_barrier();
         uint32_t lost = 0;
         T ma[EPC], ba[EPC], mb[EPC], bb[EPC];
         size_load_chunk<T>(mask, bitval, 0, ma, ba);
-        uint4 va = rowq[0], vb;
+        uint4 va = rowq[sw], vb;
         int c = 0;
         for (; c + 1 < C; c += 2) {
             size_need_chunk<T>(ma, ba, va);
             size_load_chunk<T>(mask, bitval, c + 1, mb, bb);
-            vb = rowq[c + 1];
+            vb = rowq[(c + 1) ^ sw];
             size_score_chunk<T>(va, ma, ba, lost);
             size_need_chunk<T>(mb, bb, vb);
             const int cn = c + 2 < C ? c + 2 : c + 1;
             size_load_chunk<T>(mask, bitval, cn, ma, ba);
-            va = rowq[cn];
+            va = rowq[cn ^ sw];
             size_score_chunk<T>(vb, mb, bb, lost);
         }
         if (c < C) { // odd C: the last chunk
@@ -604,7 +606,8 @@ __global__ __launch_bounds__(256) void pack_small_kernel(PackParams p)
 // constants are uniform scalar loads, a chunk ahead.  In place is safe: after
 // element j at most j + 1 payload words are complete, so the words written
 // while chunk c is packed end inside chunk c + 1 (the header is at most one
-// chunk), which was read before chunk c's first write.
+// chunk), which was read before chunk c's first write; a raw row's words past
+// its end stay in its own last word.
 //
 // Compared with a wave per row (pack_small_kernel: a wave scan, a ballot and
 // LDS atomics per row, about 78 vector and 76 scalar instructions per row on
@@ -721,10 +724,13 @@ struct RowPacker {
 };
 
 // Packs the EPC elements of chunk c (16 bytes v) of a row: completed words go
-// to row[wpos], or to the lane's dummy word when none completes (no branch).
+// to row[wpos ^ swe] (swe: the row's chunk XOR in words), or to the lane's
+// dummy word when none completes (no branch).  A row whose words reach V is
+// raw and never written back: its words past V - 1 go to its own last word,
+// so no lane writes into another lane's row.
 template <typename T, int R>
-__device__ __forceinline__ void pack_chunk(const uint4 &v, const PackChunkCols<T, R> &k, int c, T *row, T *dummy,
-                                           RowPacker<T> &st)
+__device__ __forceinline__ void pack_chunk(const uint4 &v, const PackChunkCols<T, R> &k, int c, T *row, int swe,
+                                           int vlast, T *dummy, RowPacker<T> &st)
 {
     constexpr int B = Bits<T>::B;
     constexpr int EPC = PackChunkCols<T, R>::EPC;
@@ -747,7 +753,7 @@ __device__ __forceinline__ void pack_chunk(const uint4 &v, const PackChunkCols<T
         st.acc |= T(val >> st.nb);
         const int t = st.nb + L;
         const bool full = t >= B;
-        *(full ? row + st.wpos : dummy) = st.acc;
+        *(full ? row + ((st.wpos < vlast ? st.wpos : vlast) ^ swe) : dummy) = st.acc;
         // val << (B - nb), which is 0 for nb = 0
         const T rem = T(T(val << 1) << (B - 1 - st.nb));
         st.acc = full ? rem : st.acc;
@@ -779,11 +785,14 @@ __device__ __forceinline__ void pack_tile_body(const PackTileParams &p, T *__res
     T *const row = reinterpret_cast<T *>(tile) + (size_t)lane * V;
     const uint4 *rowq = reinterpret_cast<const uint4 *>(row);
     T *const dummy = dummies + lane;
+    const bool swz = tile_swizzled((int64_t)V * W);
+    const int sw = tile_swz(lane, swz); // chunk c of this lane's row sits at c ^ sw
+    const int swe = sw * EPC;           // ... and word w at w ^ swe
     // write-back walk: chunk g = lane + 64 i of the tile is (g / C, g % C)
     const int q64 = kWave / C, r64 = kWave % C;
     const int wr0 = lane / C, wc0 = lane - (lane / C) * C;
     for (int64_t tl = blockIdx.x; tl < ntiles; tl += gridDim.x) {
-        copy_tile<T, 1>(data, p.num_vecs, V, C, tl, 0, lane, tile);
+        copy_tile<T, 1>(data, p.num_vecs, V, C, tl, 0, lane, tile, swz);
         __builtin_amdgcn_s_waitcnt(0x0f70); // vmcnt(0): the tile has landed
         __builtin_amdgcn_wave_barrier();
         const int64_t r = tl * kWave + lane;
@@ -794,27 +803,27 @@ __device__ __forceinline__ void pack_tile_body(const PackTileParams &p, T *__res
         st.fl = 0;
         PackChunkCols<T, R> ka, kb;
         load_pack_cols<T, R>(cols, ka);
-        uint4 va = rowq[0], vb;
+        uint4 va = rowq[sw], vb;
         int c = 0;
         for (; c + 1 < C; c += 2) {
             need_pack_cols<T, R>(ka);
             load_pack_cols<T, R>(cols + (size_t)(c + 1) * EPC, kb);
-            vb = rowq[c + 1];
-            pack_chunk<T, R>(va, ka, c, row, dummy, st);
+            vb = rowq[(c + 1) ^ sw];
+            pack_chunk<T, R>(va, ka, c, row, swe, V - 1, dummy, st);
             need_pack_cols<T, R>(kb);
             load_pack_cols<T, R>(cols + (size_t)(c + 2) * EPC, ka); // chunk C is the zero one
-            va = rowq[c + 2 < C ? c + 2 : c + 1];
-            pack_chunk<T, R>(vb, kb, c + 1, row, dummy, st);
+            va = rowq[(c + 2 < C ? c + 2 : c + 1) ^ sw];
+            pack_chunk<T, R>(vb, kb, c + 1, row, swe, V - 1, dummy, st);
         }
         if (c < C) { // odd C: the last chunk
             need_pack_cols<T, R>(ka);
-            pack_chunk<T, R>(va, ka, c, row, dummy, st);
+            pack_chunk<T, R>(va, ka, c, row, swe, V - 1, dummy, st);
         }
-        *(st.nb > 0 ? row + st.wpos : dummy) = st.acc; // the last, partial word
+        *(st.nb > 0 ? row + ((st.wpos < V - 1 ? st.wpos : V - 1) ^ swe) : dummy) = st.acc; // the last, partial word
         st.wpos += st.nb > 0 ? 1 : 0;
         // header: flag j in word j / B, LSB first (at most two words)
-        row[0] = (T)st.fl;
-        if (H > 1) row[1] = (T)(st.fl >> 32);
+        row[0 ^ swe] = (T)st.fl;
+        if (H > 1) row[1 ^ swe] = (T)(st.fl >> 32);
         const bool packed = r < p.num_vecs && st.wpos < V;
         rowlen[lane] = packed ? st.wpos : 0;
         const uint64_t pk = ballot(packed);
@@ -835,7 +844,8 @@ __device__ __forceinline__ void pack_tile_body(const PackTileParams &p, T *__res
             const int len = rowlen[wr];
             if (len > 0) {
                 uint4 v = reinterpret_cast<const uint4 *>(tile)[g];
-                const int w0 = wc * EPC; // first word of the chunk
+                const int ws = tile_swz(wr, swz); // LDS chunk g is chunk wc ^ ws of row wr
+                const int w0 = (wc ^ ws) * EPC;   // first word of the chunk
                 if constexpr (W == 8) {
                     if (w0 >= len) v.x = v.y = 0u;
                     if (w0 + 1 >= len) v.z = v.w = 0u;
@@ -845,7 +855,7 @@ __device__ __forceinline__ void pack_tile_body(const PackTileParams &p, T *__res
                     if (w0 + 2 >= len) v.z = 0u;
                     if (w0 + 3 >= len) v.w = 0u;
                 }
-                out[g] = v;
+                out[g ^ ws] = v;
             }
             wr += q64;
             wc += r64;
@@ -1064,6 +1074,10 @@ int64_t *noalloc_scratch()
     return g_scratch_ptr[dev];
 }
 
+// LDS bytes of one pack_tile_kernel block: the tile, the rows' word counts
+// and the lanes' dummy words.
+inline size_t pack_tile_lds(int64_t V, int w) { return (size_t)kWave * V * w + kWave * sizeof(int) + kWave * w; }
+
 // In-place pack of a table of rows of <= 64 elements in 64-row tiles (see
 // pack_tile_kernel); the column records live in a stream-ordered allocation.
 template <typename T>
@@ -1088,7 +1102,7 @@ int pack_tile_launch(const PackParams &p, hipStream_t stream)
         tp.V = V;
         tp.max_runs = max_runs;
         tp.bitmask = p.bitmask;
-        const size_t lds = (size_t)kWave * V * W + kWave * sizeof(int) + kWave * W;
+        const size_t lds = pack_tile_lds(V, W);
         int64_t per_cu = (int64_t)(160 * 1024) / (int64_t)lds;
         if (per_cu > 8) per_cu = 8;
         if (per_cu < 1) per_cu = 1;
@@ -1112,10 +1126,13 @@ int pack_typed(PackParams p, hipStream_t stream)
     const int64_t lds_budget = 64 * 1024;
     const int64_t ngroups = (p.n + 31) / 32;
     if constexpr (sizeof(T) >= 4 && !CONDENSED) {
-        // in place, no index, one mask, rows the LDS tiles take: a lane per row
+        // in place, no index, one mask, rows the LDS tiles take, six tiles
+        // per CU at least (the pack hides its latency with resident waves:
+        // 512-byte rows, four tiles per CU, pack slower than with a wave per
+        // row): a lane per row
         if (p.in == (const void *)p.out && p.index == nullptr && p.clusters == nullptr && p.bitmask != nullptr &&
             p.n == p.num_vecs && p.n > 0 && p.V <= kWave && tile_rows_ok(p.V * (int64_t)sizeof(T)) &&
-            ((uintptr_t)p.in % 16) == 0)
+            pack_tile_lds(p.V, (int)sizeof(T)) * 6 <= 160 * 1024 && ((uintptr_t)p.in % 16) == 0)
             return pack_tile_launch<T>(p, stream);
     }
     if (ngroups < (int64_t)dev.num_cus * 8) {

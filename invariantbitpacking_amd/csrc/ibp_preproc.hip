@@ -466,6 +466,8 @@ __global__ __launch_bounds__(64 * WPT) void score_tile_kernel(const T *__restric
     const size_t tile_bytes = (size_t)C * kWave * 16;
     const int64_t ntiles = (num_vecs + kWave - 1) / kWave;
     const int64_t nblocks = gridDim.x;
+    const bool swz = tile_swizzled((int64_t)C * 16);
+    const int sw = tile_swz(lane, swz); // chunk c of this lane's row sits at c ^ sw
     const ScoreRec<T, TPW, NB> *rec = recs + (size_t)wib * (C + 1);
     uint32_t pcsum[TPW];
     unsigned long long saved[TPW];
@@ -483,16 +485,16 @@ __global__ __launch_bounds__(64 * WPT) void score_tile_kernel(const T *__restric
     const bool active = t0 < nt; // waves past the last threshold only copy
     int cur = 0;
     int64_t tl = blockIdx.x;
-    if (DB && tl < ntiles) copy_tile<T, WPT>(data, num_vecs, V, C, tl, wib, lane, tiles);
+    if (DB && tl < ntiles) copy_tile<T, WPT>(data, num_vecs, V, C, tl, wib, lane, tiles, swz);
     for (; tl < ntiles; tl += nblocks) {
-        if (!DB) copy_tile<T, WPT>(data, num_vecs, V, C, tl, wib, lane, tiles);
+        if (!DB) copy_tile<T, WPT>(data, num_vecs, V, C, tl, wib, lane, tiles, swz);
         __builtin_amdgcn_s_waitcnt(0x0f70); // vmcnt(0): this wave's part of the tile has landed
         __syncthreads();                    // ... and every other wave's part
         const unsigned char *buf = tiles + (size_t)cur * tile_bytes;
         // the next tile goes to the other buffer, read by the previous
         // iteration (finished at the barrier above)
         if (DB && tl + nblocks < ntiles)
-            copy_tile<T, WPT>(data, num_vecs, V, C, tl + nblocks, wib, lane, tiles + (size_t)(cur ^ 1) * tile_bytes);
+            copy_tile<T, WPT>(data, num_vecs, V, C, tl + nblocks, wib, lane, tiles + (size_t)(cur ^ 1) * tile_bytes, swz);
         if (active) {
             const uint4 *mine = reinterpret_cast<const uint4 *>(buf) + lane * C;
             const int64_t row = tl * kWave + lane;
@@ -522,18 +524,18 @@ __global__ __launch_bounds__(64 * WPT) void score_tile_kernel(const T *__restric
                 }
             };
             load_rec<T, TPW, NB>(rec, ma, pa, ba);
-            uint4 va = mine[0], vb;
+            uint4 va = mine[sw], vb;
             int c = 0;
             for (; c + 1 < C; c += 2) {
                 need_rec<T, TPW, NB>(ma, pa, ba);
                 asm volatile("" ::"v"(va.x), "v"(va.y), "v"(va.z), "v"(va.w));
                 load_rec<T, TPW, NB>(rec + c + 1, mb, pb, bb);
-                vb = mine[c + 1];
+                vb = mine[(c + 1) ^ sw];
                 score(va, ma, pa, ba);
                 need_rec<T, TPW, NB>(mb, pb, bb);
                 asm volatile("" ::"v"(vb.x), "v"(vb.y), "v"(vb.z), "v"(vb.w));
                 load_rec<T, TPW, NB>(rec + c + 2, ma, pa, ba); // record C is the zero one
-                va = mine[c + 2 < C ? c + 2 : c + 1];
+                va = mine[(c + 2 < C ? c + 2 : c + 1) ^ sw];
                 score(vb, mb, pb, bb);
             }
             if (c < C) { // odd C: the last chunk

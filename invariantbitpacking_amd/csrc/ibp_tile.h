@@ -8,30 +8,42 @@
 
 namespace ibp {
 
-// Rows of up to 512 bytes, whole 16-byte chunks, whose stride in dwords is
-// not a multiple of 16 modulo 32: lane r reading 16 bytes of row r then
-// spreads eight lanes over the LDS banks with at most two hits per bank group.
+// Rows of up to 512 bytes of whole 16-byte chunks.  Lane r reads row r of
+// the tile 16 bytes at a time; eight lanes' reads share the LDS banks once
+// each when the row stride spreads them.  A stride that is a multiple of 128
+// bytes would put all of them in one bank group, so there chunk c of tile row
+// r is stored at position c ^ (r & 7) of its row (a permutation inside each
+// 128-byte group of the row: every copy instruction still reads 1 KB of
+// consecutive bytes).  Strides of 64 bytes modulo 128 stay out (four lanes per
+// bank group).
+__host__ __device__ constexpr bool tile_swizzled(int64_t row_bytes) { return row_bytes % 128 == 0; }
+
 constexpr int kTileMaxRowBytes = 512;
 __host__ __device__ constexpr bool tile_rows_ok(int64_t row_bytes)
 {
-    return row_bytes % 16 == 0 && row_bytes <= kTileMaxRowBytes && (row_bytes / 4) % 32 != 0 &&
-           (row_bytes / 4) % 32 != 16;
+    return row_bytes % 16 == 0 && row_bytes <= kTileMaxRowBytes && row_bytes % 128 != 64;
 }
+
+// The XOR applied to the chunk index of tile row r (0 without the swizzle).
+__device__ __forceinline__ int tile_swz(int r, bool swz) { return swz ? (r & 7) : 0; }
 
 // The copy of tile tl (rows tl*64 ...) into `buf` by wave `wib` of WPT: 16-byte
 // chunks wib, wib + WPT, ... of the tile, each instruction 1 KB of consecutive
 // bytes straight to LDS (global_load_lds_dwordx4: no registers, tracked by
-// vmcnt).  C = row bytes / 16.
+// vmcnt).  C = row bytes / 16; swz: the placement above (then C % 8 == 0, and
+// the lane of LDS chunk g loads chunk g ^ (row & 7) of the same row).
 template <typename T, int WPT>
 __device__ __forceinline__ void copy_tile(const T *data, int64_t num_vecs, int V, int C, int64_t tl, int wib, int lane,
-                                          unsigned char *buf)
+                                          unsigned char *buf, bool swz)
 {
     const uint4 *src = reinterpret_cast<const uint4 *>(data + tl * kWave * (int64_t)V);
     const int64_t rows_left = num_vecs - tl * kWave;
     // the last tile may be short: its missing rows re-read the last chunk
     const int64_t last = (rows_left < kWave ? rows_left : kWave) * C - 1;
+    const float rc = 1.0f / (float)C;
     for (int k = wib; k < C; k += WPT) {
-        const int64_t g = (int64_t)k * kWave + lane;
+        int g = k * kWave + lane;
+        if (swz) g ^= (int)(((float)g + 0.5f) * rc) & 7; // row g / C (exact: g < 2048, C <= 32)
         __builtin_amdgcn_global_load_lds(static_cast<const void *>(src + (g < last ? g : last)),
                                          (__attribute__((address_space(3))) void *)(buf + (size_t)k * (kWave * 16)),
                                          16, 0, 0);

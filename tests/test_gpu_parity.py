@@ -172,11 +172,12 @@ def masked_rows(n, v, w, seed, kind, flag_frac=0.8):
     return (x.astype(dt).view(np.dtype("i%d" % w)), m.astype(dt), b.astype(dt))
 
 
-# Rows of whole 16-byte chunks whose stride spreads over the LDS banks take
-# the row-per-lane tile size and pack kernels; the shapes cover one and an odd
-# number of chunks, one- and two-word headers and short last tiles.
+# Rows of whole 16-byte chunks (not 64 bytes modulo 128) take the row-per-lane
+# tile size and pack kernels; the shapes cover one and an odd number of
+# chunks, one- and two-word headers, short last tiles, and rows of a multiple
+# of 128 bytes, whose chunks are permuted in LDS (128, 256, 384, 512 bytes).
 TILE_SHAPES = [(500, 50, 8), (300, 2, 8), (1000, 62, 8), (777, 12, 4), (2000, 60, 4), (130, 36, 4), (64, 4, 4),
-               (70000, 50, 8)]
+               (70000, 50, 8), (600, 64, 8), (300, 16, 8), (513, 48, 8), (700, 64, 4), (200, 32, 4)]
 
 
 @pytest.mark.parametrize("shape", TILE_SHAPES)
