@@ -804,8 +804,8 @@ def run_gnn_features(args, calls=64):
     of 4096 random rows, latency-bound.  For the table in HBM and in pinned
     memory: the isolated call's wall latency, `calls` back-to-back plain
     calls (HIP events), the same `calls` index sets fetched by ONE
-    graph-replayed batched launch, and the host time of one call (Python
-    checks + ctypes, no GPU wait).  Every output is checked."""
+    graph-replayed batched launch, and the host time of one call (the
+    Python call and the C++ binding's checks and launch, no GPU wait).  Every output is checked."""
     import invariantbitpacking_amd as ibp
     from invariantbitpacking_amd import workloads
     x, _ = workloads.gnn_features()

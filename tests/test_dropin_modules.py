@@ -38,6 +38,36 @@ def positional(fn):
             if p.kind in (p.POSITIONAL_ONLY, p.POSITIONAL_OR_KEYWORD)]
 
 
+def binding_params(fn):
+    """(name, has_default, default text) of each positional parameter of a
+    compiled (pybind11) binding, read from the signature line pybind11 writes
+    at the top of its docstring; keyword-only parameters (after `*`) are left
+    out."""
+    line = fn.__doc__.strip().splitlines()[0]
+    inner = line[line.index("(") + 1:line.rindex(")")]
+    parts, depth, cur = [], 0, ""
+    for ch in inner:
+        if ch in "[(":
+            depth += 1
+        elif ch in "])":
+            depth -= 1
+        if ch == "," and depth == 0:
+            parts.append(cur.strip())
+            cur = ""
+        else:
+            cur += ch
+    if cur.strip():
+        parts.append(cur.strip())
+    out = []
+    for part in parts:
+        if part == "*":
+            break
+        name = part.split(":")[0].strip()
+        default = part.split("=", 1)[1].strip() if "=" in part else None
+        out.append((name, default is not None, default))
+    return out
+
+
 def test_ibp_module_names_and_order():
     import ibp
     for name, params in IBP_MODULE.items():
@@ -54,13 +84,28 @@ def test_ibp_module_names_and_order():
 
 
 def test_ibp_cuda_module_bindings():
+    """ibp_cuda's functions are compiled bindings (a C++ torch extension) with
+    the reference's positional arity and optional trailing arguments; one
+    positional argument more is a TypeError, as for the reference's bindings."""
+    import torch
+
     import ibp_cuda
-    assert sorted(ibp_cuda.__all__) == sorted(IBP_CUDA_ARITY)
+    assert set(IBP_CUDA_ARITY) <= set(ibp_cuda.__all__)
+    assert set(ibp_cuda.__all__) - set(IBP_CUDA_ARITY) == {"decompress"}  # declared, never built by the reference
     for name, (arity, optional) in IBP_CUDA_ARITY.items():
-        got = positional(getattr(ibp_cuda, name))
-        assert len(got) == arity, name
-        assert sum(p.default is None for p in got) == optional, name
-        assert all(p.default is None for p in got[arity - optional:]), name
+        fn = getattr(ibp_cuda, name)
+        assert type(fn).__name__ == "builtin_function_or_method", name
+        got = binding_params(fn)
+        assert len(got) == arity, (name, got)
+        assert [d for _, d, _ in got] == [False] * (arity - optional) + [True] * optional, (name, got)
+        assert all(text == "None" for _, d, text in got if d), (name, got)
+        with pytest.raises(TypeError):
+            fn(*([None] * (arity + 1)))
+    # the reference's positional call of decompress_fetch reaches the argument checks
+    x = torch.zeros((4, 3), dtype=torch.int64)
+    with pytest.raises(RuntimeError, match="accessible by CUDA device"):
+        ibp_cuda.decompress_fetch(x, x[0], x[0], torch.zeros(1, dtype=torch.int32), torch.device("cuda"), None, None,
+                                  None, None, None, None)
 
 
 def test_modules_forward_to_the_library():
