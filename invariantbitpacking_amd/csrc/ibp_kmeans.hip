@@ -67,7 +67,8 @@ __global__ __launch_bounds__(kWave * kAssignWaves) void km_assign_kernel(const T
     const int lane = lane_id();
     const int wv = wave_in_block_id();
     const int sl = wv % SPLIT; // this wave's column slice
-    const int rr = lane % kAssignRows, gl = lane / kAssignRows; // the (row, centroid) lane l ends with
+    // lane l ends with the distance of row rr = l % kAssignRows to centroid gl of the group
+    const int gl = lane / kAssignRows;
     for (int64_t i0 = ((int64_t)blockIdx.x * GPB + wv / SPLIT) * kAssignRows; i0 < N;
          i0 += (int64_t)gridDim.x * GPB * kAssignRows) {
         int best = 0, best_d = 0x7fffffff; // of row rr

@@ -346,21 +346,61 @@ __global__ __launch_bounds__(1024) void pack_kernel(PackParams p)
             const T *bitval = (const T *)p.bitval + c * V;
             T xs = T(0);
             bool fs = false;
-            int saved;
             int incl = 0; // small rows: inclusive scan of the payload lengths
+            int64_t size;
             if (small) {
                 xs = col.valid ? src[lane] : T(0);
                 fs = col.valid && (T)(xs & T(~col.free)) == col.bitval;
                 // The payload scan gives the size as well: the bits saved are
                 // V*B minus the payload length.
                 incl = wave_incl_scan(fs ? col.len_flagged : len_raw);
-                saved = (int)V * B - readlane(incl, kWave - 1);
+                size = slot_size(V, W, (int)V * B - readlane(incl, kWave - 1));
             } else {
-                saved = row_saved_bits<T>(src, mask, bitval, V);
+                // Wide rows are packed into the slot buffer in one pass over
+                // the row; the size follows from the payload length, and a row
+                // that does not come out smaller is dropped afterwards.  So
+                // every row is read once (a raw row copied out of place twice),
+                // which halves the reads of host-resident tables over PCIe.
+                if constexpr (!XCHG) {
+                    for (int64_t k = lane; k < buf_bytes / 4; k += kWave) ((uint32_t *)bufb)[k] = 0u;
+                    slot_fence<BUF_LDS>();
+                }
+                int64_t cursor = 0;
+                for (int64_t c0 = 0; c0 < V; c0 += kWave) {
+                    const int64_t j = c0 + lane;
+                    const bool valid = j < V;
+                    const T x = valid ? src[j] : T(0);
+                    const T m = valid ? mask[j] : T(0);
+                    const bool f = valid && (T)(x & m) == (valid ? bitval[j] : T(0));
+                    // Header flags of this chunk, LSB first (ibp_compress_dev.cuh:60-71):
+                    // 64 / B header words, word k holding flags k*B .. k*B+B-1.
+                    const uint64_t hb = ballot(f);
+                    if (lane < kWave / B && c0 + lane * B < V) {
+                        const T hw = (T)(hb >> (lane * B));
+                        if (hw) or_word<T>(buf, c0 / B + lane, hw);
+                    }
+                    const int L = valid ? (f ? B - popc(m) : B) : 0;
+                    const int incl = wave_incl_scan(L);
+                    const int tot = readlane(incl, kWave - 1);
+                    const int64_t pbit = cursor + incl - L;
+                    // Payload: free bits MSB first (flagged) or the whole element.
+                    // A packed row ends before word V; words from V on belong
+                    // to a row that will be dropped and stay unwritten (the
+                    // buffer holds V words).
+                    const T v = f ? extract_msb<T>(x, T(~m)) : x;
+                    const int64_t q = Hw + pbit / B;
+                    const int o = (int)(pbit % B);
+                    if (L > 0 && q < V) {
+                        or_word<T>(buf, q, shr<T>(v, o));
+                        if (o + L > B && q + 1 < V) or_word<T>(buf, q + 1, shl<T>(v, B - o));
+                    }
+                    cursor += tot;
+                }
+                size = slot_size(V, W, (int)(V * B - cursor));
             }
             // Size test (ibp_compress_kernel.cuh:28-31): pack only if smaller.
-            const int64_t size = slot_size(V, W, saved);
-            T *dst;
+            T *dst = nullptr;
+            bool row_ok = true;
             if constexpr (CONDENSED) {
                 const int64_t lo = i > 0 ? readfirstlane(p.offsets[i - 1]) : 0;
                 const int64_t hi = readfirstlane(p.offsets[i]);
@@ -368,14 +408,20 @@ __global__ __launch_bounds__(1024) void pack_kernel(PackParams p)
                 // another dataset or mask never make a write leave the row.
                 if (lo < 0 || hi - lo != size || hi > p.stream_bytes || (lo % W) != 0) {
                     ++bad;
-                    continue;
+                    row_ok = false;
+                } else {
+                    dst = (T *)((unsigned char *)p.out + lo);
                 }
-                dst = (T *)((unsigned char *)p.out + lo);
             } else {
                 dst = (T *)p.out + i * V;
             }
-            if (size >= vec_bytes) {
-                if (!in_place) {
+            if (!row_ok || size >= vec_bytes) {
+                if (XCHG && !small) { // undo the dropped row's words: the buffer stays zero
+                    slot_fence<BUF_LDS>();
+                    for (int64_t k = lane; k < buf_bytes / 16; k += kWave) ((uint4 *)bufb)[k] = make_uint4(0u, 0u, 0u, 0u);
+                    slot_fence<BUF_LDS>();
+                }
+                if (row_ok && !in_place) {
                     if (small) {
                         if (col.valid) dst[lane] = xs;
                     } else {
@@ -384,12 +430,11 @@ __global__ __launch_bounds__(1024) void pack_kernel(PackParams p)
                 }
                 continue;
             }
-            if constexpr (!XCHG) {
-                for (int64_t k = lane; k < buf_bytes / 4; k += kWave) ((uint32_t *)bufb)[k] = 0u;
-                slot_fence<BUF_LDS>();
-            }
-            int64_t cursor = 0;
             if (small) {
+                if constexpr (!XCHG) {
+                    for (int64_t k = lane; k < buf_bytes / 4; k += kWave) ((uint32_t *)bufb)[k] = 0u;
+                    slot_fence<BUF_LDS>();
+                }
                 const uint64_t hb = ballot(fs);
                 if (lane < kWave / B && lane * B < V) {
                     const T hw = (T)(hb >> (lane * B));
@@ -404,33 +449,6 @@ __global__ __launch_bounds__(1024) void pack_kernel(PackParams p)
                     or_word<T>(buf, q, shr<T>(v, o));
                     if (o + L > B) or_word<T>(buf, q + 1, shl<T>(v, B - o));
                 }
-            }
-            for (int64_t c0 = 0; !small && c0 < V; c0 += kWave) {
-                const int64_t j = c0 + lane;
-                const bool valid = j < V;
-                const T x = valid ? src[j] : T(0);
-                const T m = valid ? mask[j] : T(0);
-                const bool f = valid && (T)(x & m) == (valid ? bitval[j] : T(0));
-                // Header flags of this chunk, LSB first (ibp_compress_dev.cuh:60-71):
-                // 64 / B header words, word k holding flags k*B .. k*B+B-1.
-                const uint64_t hb = ballot(f);
-                if (lane < kWave / B && c0 + lane * B < V) {
-                    const T hw = (T)(hb >> (lane * B));
-                    if (hw) or_word<T>(buf, c0 / B + lane, hw);
-                }
-                const int L = valid ? (f ? B - popc(m) : B) : 0;
-                const int incl = wave_incl_scan(L);
-                const int tot = readlane(incl, kWave - 1);
-                const int64_t pbit = cursor + incl - L;
-                // Payload: free bits MSB first (flagged) or the whole element.
-                const T v = f ? extract_msb<T>(x, T(~m)) : x;
-                const int64_t q = Hw + pbit / B;
-                const int o = (int)(pbit % B);
-                if (L > 0) {
-                    or_word<T>(buf, q, shr<T>(v, o));
-                    if (o + L > B) or_word<T>(buf, q + 1, shl<T>(v, B - o));
-                }
-                cursor += tot;
             }
             slot_fence<BUF_LDS>();
             const int64_t words = size / W; // the rest of the slot is zero

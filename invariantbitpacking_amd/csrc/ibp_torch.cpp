@@ -91,7 +91,13 @@ bool host_temp(const OptTensor &orig, const Tensor &used)
 
 void *cur_stream() { return c10::hip::getCurrentHIPStream().stream(); }
 
-void sync_stream() { c10::hip::getCurrentHIPStream().synchronize(); }
+// Waits for the current stream with the GIL released (other Python threads
+// keep running, as they do while a ctypes call waits).
+void sync_stream()
+{
+    py::gil_scoped_release nogil;
+    c10::hip::getCurrentHIPStream().synchronize();
+}
 
 at::ScalarType signed_type(int64_t w)
 {
@@ -271,9 +277,14 @@ std::tuple<Tensor, Tensor, Tensor> preprocess_kmeans(const Tensor &dataset, int6
     Tensor bitvals = at::zeros({num_clusters, v}, on_gpu(signed_type(w)));
     Tensor clusters = pinned_empty({n}, at::kInt).zero_();
     const float t = threshold.has_value() ? (float)*threshold : -1.0f;
-    check_status(ibp_preprocess_kmeans((int)w, ptr(dataset), n, v, (int)num_clusters, t,
-                                       seed.has_value() ? *seed : -1, ptr(masks), ptr(bitvals),
-                                       static_cast<int32_t *>(clusters.data_ptr()), cur_stream()));
+    void *st = cur_stream();
+    int rc;
+    {
+        py::gil_scoped_release nogil;  // the call synchronises its stream
+        rc = ibp_preprocess_kmeans((int)w, ptr(dataset), n, v, (int)num_clusters, t, seed.has_value() ? *seed : -1,
+                                   ptr(masks), ptr(bitvals), static_cast<int32_t *>(clusters.data_ptr()), st);
+    }
+    check_status(rc);
     return {masks, bitvals, clusters};
 }
 
@@ -475,7 +486,13 @@ Tensor read_shared(const std::string &filename, const std::vector<int64_t> &shap
         t = at::empty({0}, at::TensorOptions().dtype(type)).set_(storage, 0, shape);
     }
     size_t got = 0;
-    check_status(ibp_read_file(filename.c_str(), nbytes ? t.data_ptr() : nullptr, nbytes, (int)threads, &got));
+    void *dst = nbytes ? t.data_ptr() : nullptr;
+    int rc;
+    {
+        py::gil_scoped_release nogil;
+        rc = ibp_read_file(filename.c_str(), dst, nbytes, (int)threads, &got);
+    }
+    check_status(rc);
     return t;
 }
 
@@ -484,7 +501,12 @@ Tensor read_shared(const std::string &filename, const std::vector<int64_t> &shap
 uint64_t index_errors(bool reset)
 {
     uint64_t v = 0;
-    check_status(ibp_index_errors(&v, reset ? 1 : 0));
+    int rc;
+    {
+        py::gil_scoped_release nogil;  // synchronises the device
+        rc = ibp_index_errors(&v, reset ? 1 : 0);
+    }
+    check_status(rc);
     return v;
 }
 
