@@ -137,6 +137,56 @@ def test_size_and_compress_match_oracle(ibp, oracle, w, shape):
         assert np.array_equal(_u(_np(t)), _u(ocomp)), "slots"
 
 
+@pytest.mark.parametrize("w", [1, 2, 4, 8])
+@pytest.mark.parametrize("v", [65, 130, 300, 1024])
+def test_wide_rows_mixing_raw_and_packed_match_oracle(ibp, oracle, w, v):
+    """Wide rows are packed into the slot buffer before their size is known
+    and dropped when they come out raw.  Rows that pack and rows of random
+    bits (raw) interleave here, so every raw row's speculative pack runs
+    between packed rows of the same and of neighbouring waves: slots,
+    bitmask, condensed stream and sizes must equal the oracle's for the
+    in-place pack, the indexed pack, the condensed stream and clusters."""
+    n = 480
+    rng = np.random.default_rng(1000 + v + w)
+    data, om, ob = masked_rows(n, v, w, seed=v * 10 + w, kind="runs2", flag_frac=0.97)
+    raw = rng.random(n) < 0.4
+    noise = rng.integers(0, 256, size=(n, v * w), dtype=np.uint8).view(data.dtype)
+    data = np.ascontiguousarray(np.where(raw[:, None], noise, data))
+    mask = torch.from_numpy(om.view(data.dtype)).cuda()
+    bitval = torch.from_numpy(ob.view(data.dtype)).cuda()
+    osz, _ = oracle.get_compress_size(data, om, ob)
+    assert (osz == v * w).any() and (osz < v * w).any()  # both kinds present
+    print("wide mixed", w, v, flush=True)
+    t = place(data, "device")
+    assert np.array_equal(_np(ibp.get_compress_size(t, mask, bitval)), osz)
+    ocomp, obm = oracle.compress(data, om, ob)
+    bm = ibp.compress_inplace(t, mask, bitval)
+    torch.cuda.synchronize()
+    assert np.array_equal(_np(bm), obm), "bitmask"
+    assert np.array_equal(_u(_np(t)), _u(ocomp)), "slots"
+    sel = rng.integers(0, n, size=300)
+    ocomp_i, obm_i = oracle.compress(data, om, ob, index=sel)
+    t = place(data, "device")
+    bm = ibp.compress_inplace(t, mask, bitval, torch.from_numpy(sel).cuda())
+    torch.cuda.synchronize()
+    assert np.array_equal(_np(bm), obm_i), "bitmask (index)"
+    assert np.array_equal(_u(_np(t)[:len(sel)]), _u(ocomp_i)), "slots (index)"
+    ws, wo = oracle.compress_condensed(data, om, ob, index=sel)
+    stream, off = ibp.compress_condensed(place(data, "device"), mask, bitval, torch.from_numpy(sel).cuda())
+    torch.cuda.synchronize()
+    assert np.array_equal(_np(off), wo) and np.array_equal(_u(_np(stream)), _u(ws)), "condensed"
+    # two mask rows, the second never matching: rows of cluster 1 are raw
+    masks2 = np.stack([om, om]).view(data.dtype)
+    bitvals2 = np.stack([ob, ~ob]).view(data.dtype)
+    cl = rng.integers(0, 2, size=n).astype(np.int32)
+    oc, obmc = oracle.compress(data, masks2, bitvals2, clusters=cl)
+    t = place(data, "device")
+    bm = ibp.compress_inplace(t, torch.from_numpy(masks2).cuda(), torch.from_numpy(bitvals2).cuda(),
+                              clusters=torch.from_numpy(cl).cuda())
+    torch.cuda.synchronize()
+    assert np.array_equal(_np(bm), obmc) and np.array_equal(_u(_np(t)), _u(oc)), "clusters"
+
+
 def masked_rows(n, v, w, seed, kind, flag_frac=0.8):
     """Rows drawn against a chosen mask/bitval so that about flag_frac of the
     elements are flagged.  kind picks the free bits of each column: 'runsR'
