@@ -885,7 +885,9 @@ def run_kernels(args):
     rocprofv3 (each op `--steps` times, HIP events around each):
     preprocess (sweep), get_compress_size, compress_inplace, the plain fetch of
     200,000 random rows from HBM, the raw-row gather (impl 2) of the same rows
-    from the UNCOMPRESSED table, torch's x[idx] gather, a 256 MiB device copy
+    from the UNCOMPRESSED table, torch's x[idx] gather, compress_condensed of
+    the whole table and the fetch of the same rows from its stream, a 256 MiB
+    device copy
     (the counter calibration for HBM), the same fetch and raw gather from
     pinned memory, and a sequential read of a 256 MiB pinned table (every row
     in order, impl 2: the calibration for host-read counters)."""
@@ -942,6 +944,16 @@ def run_kernels(args):
     ev("raw_gather_hbm", lambda: ibp.decompress_fetch(x, mask, bitval, zero, dev, v, idx, None, None, 2, out), dec)
     ok = ok and (ops is None or "raw_gather_hbm" not in ops or bool(torch.equal(out, x[idx])))
     ev("torch_index_gather_hbm", lambda: torch.index_select(x, 0, idx, out=out), dec)
+    if want("compress_condensed") or want("fetch_condensed_hbm"):
+        def cond():
+            st["cc"] = ibp.compress_condensed(x, mask, bitval)
+        ev("compress_condensed", cond, x.numel() * 8)
+        if "cc" not in st:
+            cond()
+        stream, offs = st["cc"]
+        ev("fetch_condensed_hbm", lambda: ibp.decompress(stream, offs, mask, bitval, dev, v, idx, out, cl), dec)
+        ok = ok and (not want("fetch_condensed_hbm") or bool(torch.equal(out, x[idx])))
+        del st["cc"], stream, offs
     if want("device_copy_256MiB"):
         a = torch.empty(256 << 20, dtype=torch.uint8, device="cuda")
         b = torch.empty_like(a)

@@ -745,10 +745,11 @@ struct RowPacker {
 // to row[wpos ^ swe] (swe: the row's chunk XOR in words), or to the lane's
 // dummy word when none completes (no branch).  A row whose words reach V is
 // raw and never written back: its words past V - 1 go to its own last word,
-// so no lane writes into another lane's row.
+// so no lane writes into another lane's row.  wet = false sends every word
+// to the dummy: the row is measured, not changed.
 template <typename T, int R>
 __device__ __forceinline__ void pack_chunk(const uint4 &v, const PackChunkCols<T, R> &k, int c, T *row, int swe,
-                                           int vlast, T *dummy, RowPacker<T> &st)
+                                           int vlast, T *dummy, RowPacker<T> &st, bool wet = true)
 {
     constexpr int B = Bits<T>::B;
     constexpr int EPC = PackChunkCols<T, R>::EPC;
@@ -771,7 +772,7 @@ __device__ __forceinline__ void pack_chunk(const uint4 &v, const PackChunkCols<T
         st.acc |= T(val >> st.nb);
         const int t = st.nb + L;
         const bool full = t >= B;
-        *(full ? row + ((st.wpos < vlast ? st.wpos : vlast) ^ swe) : dummy) = st.acc;
+        *(full && wet ? row + ((st.wpos < vlast ? st.wpos : vlast) ^ swe) : dummy) = st.acc;
         // val << (B - nb), which is 0 for nb = 0
         const T rem = T(T(val << 1) << (B - 1 - st.nb));
         st.acc = full ? rem : st.acc;
@@ -784,8 +785,51 @@ struct PackTileParams {
     int64_t num_vecs;
     int V;
     const int *max_runs;   // the largest run count of the column records
-    uint32_t *bitmask;
+    uint32_t *bitmask;     // may be nullptr with a condensed output
+    // condensed output only: row i goes to bytes [offsets[i-1], offsets[i])
+    const int64_t *offsets;
+    int64_t stream_bytes;
+    unsigned long long *err; // rows whose stored length is not their size
 };
+
+// Packs lane's row of the tile, in LDS at `row` (chunk c at rowq[c ^ sw],
+// word w at w ^ swe), in place: header flags, then the payload words.
+// Returns the slot's length in words, V when the row does not get smaller.
+// wet = false measures the row without changing it.
+template <typename T, int R>
+__device__ __forceinline__ int pack_lane_row(const PackCol<T> *__restrict__ cols, T *row, const uint4 *rowq, int sw,
+                                             int swe, int V, int C, int H, T *dummy, bool wet)
+{
+    RowPacker<T> st;
+    st.acc = T(0);
+    st.nb = 0;
+    st.wpos = H;
+    st.fl = 0;
+    PackChunkCols<T, R> ka, kb;
+    load_pack_cols<T, R>(cols, ka);
+    uint4 va = rowq[sw], vb;
+    int c = 0;
+    for (; c + 1 < C; c += 2) {
+        need_pack_cols<T, R>(ka);
+        load_pack_cols<T, R>(cols + (size_t)(c + 1) * PackChunkCols<T, R>::EPC, kb);
+        vb = rowq[(c + 1) ^ sw];
+        pack_chunk<T, R>(va, ka, c, row, swe, V - 1, dummy, st, wet);
+        need_pack_cols<T, R>(kb);
+        load_pack_cols<T, R>(cols + (size_t)(c + 2) * PackChunkCols<T, R>::EPC, ka); // chunk C is the zero one
+        va = rowq[(c + 2 < C ? c + 2 : c + 1) ^ sw];
+        pack_chunk<T, R>(vb, kb, c + 1, row, swe, V - 1, dummy, st, wet);
+    }
+    if (c < C) { // odd C: the last chunk
+        need_pack_cols<T, R>(ka);
+        pack_chunk<T, R>(va, ka, c, row, swe, V - 1, dummy, st, wet);
+    }
+    *(wet && st.nb > 0 ? row + ((st.wpos < V - 1 ? st.wpos : V - 1) ^ swe) : dummy) = st.acc; // the last, partial word
+    st.wpos += st.nb > 0 ? 1 : 0;
+    // header: flag j in word j / B, LSB first (at most two words)
+    *(wet ? row + (0 ^ swe) : dummy) = (T)st.fl;
+    if (H > 1) *(wet ? row + (1 ^ swe) : dummy) = (T)(st.fl >> 32);
+    return st.wpos < V ? st.wpos : V;
+}
 
 template <typename T, int R>
 __device__ __forceinline__ void pack_tile_body(const PackTileParams &p, T *__restrict__ data,
@@ -814,36 +858,9 @@ __device__ __forceinline__ void pack_tile_body(const PackTileParams &p, T *__res
         __builtin_amdgcn_s_waitcnt(0x0f70); // vmcnt(0): the tile has landed
         __builtin_amdgcn_wave_barrier();
         const int64_t r = tl * kWave + lane;
-        RowPacker<T> st;
-        st.acc = T(0);
-        st.nb = 0;
-        st.wpos = H;
-        st.fl = 0;
-        PackChunkCols<T, R> ka, kb;
-        load_pack_cols<T, R>(cols, ka);
-        uint4 va = rowq[sw], vb;
-        int c = 0;
-        for (; c + 1 < C; c += 2) {
-            need_pack_cols<T, R>(ka);
-            load_pack_cols<T, R>(cols + (size_t)(c + 1) * EPC, kb);
-            vb = rowq[(c + 1) ^ sw];
-            pack_chunk<T, R>(va, ka, c, row, swe, V - 1, dummy, st);
-            need_pack_cols<T, R>(kb);
-            load_pack_cols<T, R>(cols + (size_t)(c + 2) * EPC, ka); // chunk C is the zero one
-            va = rowq[(c + 2 < C ? c + 2 : c + 1) ^ sw];
-            pack_chunk<T, R>(vb, kb, c + 1, row, swe, V - 1, dummy, st);
-        }
-        if (c < C) { // odd C: the last chunk
-            need_pack_cols<T, R>(ka);
-            pack_chunk<T, R>(va, ka, c, row, swe, V - 1, dummy, st);
-        }
-        *(st.nb > 0 ? row + ((st.wpos < V - 1 ? st.wpos : V - 1) ^ swe) : dummy) = st.acc; // the last, partial word
-        st.wpos += st.nb > 0 ? 1 : 0;
-        // header: flag j in word j / B, LSB first (at most two words)
-        row[0 ^ swe] = (T)st.fl;
-        if (H > 1) row[1 ^ swe] = (T)(st.fl >> 32);
-        const bool packed = r < p.num_vecs && st.wpos < V;
-        rowlen[lane] = packed ? st.wpos : 0;
+        const int words = pack_lane_row<T, R>(cols, row, rowq, sw, swe, V, C, H, dummy, true);
+        const bool packed = r < p.num_vecs && words < V;
+        rowlen[lane] = packed ? words : 0;
         const uint64_t pk = ballot(packed);
         if (lane == 0) {
             p.bitmask[2 * tl] = (uint32_t)pk;
@@ -906,6 +923,104 @@ __global__ __launch_bounds__(kWave) void pack_tile_kernel(PackTileParams p, T *_
     case 3: pack_tile_body<T, 3>(p, data, cols, lds_raw, rowlen, dummies); break;
     case 4: pack_tile_body<T, 4>(p, data, cols, lds_raw, rowlen, dummies); break;
     default: pack_tile_body<T, -1>(p, data, cols, lds_raw, rowlen, dummies); break;
+    }
+}
+
+// The condensed form of pack_tile_kernel (compress_condensed of a whole table
+// with one mask, rows the LDS tiles take): the tile is packed in LDS as
+// there, except that a row whose offsets say it is stored raw is only
+// measured, so its LDS copy stays the original row.  A row is written only
+// when its stored length (offsets[i] - offsets[i-1]) is its size; otherwise
+// it is counted in p.err.  Row t's words then go to the stream at
+// offsets[t-1], four rows per step: their LDS words are read together, then
+// stored, lane k taking word k of each row (rows of <= 64 elements).
+template <typename T, int R>
+__device__ __forceinline__ void pack_tile_condensed_body(const PackTileParams &p, const T *__restrict__ data,
+                                                         T *__restrict__ stream, const PackCol<T> *__restrict__ cols,
+                                                         unsigned char *tile, T *dummies)
+{
+    constexpr int W = (int)sizeof(T);
+    constexpr int EPC = 16 / W;
+    constexpr int kRowsPerStep = 4;
+    const int lane = lane_id();
+    const int V = p.V;
+    const int C = V * W / 16;
+    const int H = (int)(header_bytes(V, W) / W);
+    const int64_t ntiles = (p.num_vecs + kWave - 1) / kWave;
+    const int64_t ngroups = (p.num_vecs + 31) / 32;
+    T *const row = reinterpret_cast<T *>(tile) + (size_t)lane * V;
+    const uint4 *rowq = reinterpret_cast<const uint4 *>(row);
+    T *const dummy = dummies + lane;
+    const bool swz = tile_swizzled((int64_t)V * W);
+    const int sw = tile_swz(lane, swz);
+    const int swe = sw * EPC;
+    int bad = 0;
+    for (int64_t tl = blockIdx.x; tl < ntiles; tl += gridDim.x) {
+        copy_tile<T, 1>(data, p.num_vecs, V, C, tl, 0, lane, tile, swz);
+        const int64_t r = tl * kWave + lane;
+        const bool in = r < p.num_vecs;
+        const int64_t lo = in && r > 0 ? p.offsets[r - 1] : 0;
+        const int64_t hi = in ? p.offsets[r] : 0;
+        __builtin_amdgcn_s_waitcnt(0x0f70); // vmcnt(0): the tile and the offsets have landed
+        __builtin_amdgcn_wave_barrier();
+        const int64_t stored = hi - lo;
+        const bool range_ok = in && lo >= 0 && hi <= p.stream_bytes && (lo % W) == 0;
+        const bool wet = range_ok && stored != (int64_t)V * W; // stored as a packed slot
+        const int words = pack_lane_row<T, R>(cols, row, rowq, sw, swe, V, C, H, dummy, wet);
+        const bool ok = range_ok && stored == (int64_t)words * W;
+        bad += in && !ok ? 1 : 0;
+        if (p.bitmask) {
+            const uint64_t pk = ballot(ok && words < V);
+            if (lane == 0) {
+                p.bitmask[2 * tl] = (uint32_t)pk;
+                if (2 * tl + 1 < ngroups) p.bitmask[2 * tl + 1] = (uint32_t)(pk >> 32);
+            }
+        }
+        const int len = ok ? words : 0;
+        const int64_t dw = lo / W; // the row's first word in the stream
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int t0 = 0; t0 < kWave; t0 += kRowsPerStep) {
+            T v[kRowsPerStep];
+            int lt[kRowsPerStep];
+            int64_t dt[kRowsPerStep];
+#pragma unroll
+            for (int u = 0; u < kRowsPerStep; ++u) {
+                const int t = t0 + u;
+                lt[u] = readlane(len, t);
+                dt[u] = readlane(dw, t);
+                const T *rt = reinterpret_cast<const T *>(tile) + (size_t)t * V;
+                v[u] = rt[(lane < lt[u] ? lane : 0) ^ (tile_swz(t, swz) * EPC)];
+            }
+#pragma unroll
+            for (int u = 0; u < kRowsPerStep; ++u)
+                if (lane < lt[u]) stream[dt[u] + lane] = v[u];
+        }
+        // every LDS read of this tile is done before the next copy lands
+        __builtin_amdgcn_s_waitcnt(0xc07f); // lgkmcnt(0)
+        __builtin_amdgcn_wave_barrier();
+    }
+    const int nbad = wave_sum(bad);
+    if (p.err && nbad && lane == 0) atomicAdd(p.err, (unsigned long long)nbad);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kWave) void pack_tile_condensed_kernel(PackTileParams p, const T *__restrict__ data,
+                                                                    T *__restrict__ stream,
+                                                                    const PackCol<T> *__restrict__ cols)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const size_t tile_bytes = (size_t)kWave * p.V * sizeof(T);
+    T *dummies = reinterpret_cast<T *>(lds_raw + tile_bytes + kWave * sizeof(int));
+    const int R = __builtin_amdgcn_readfirstlane(*p.max_runs);
+    switch (R) {
+    case 0: pack_tile_condensed_body<T, 0>(p, data, stream, cols, lds_raw, dummies); break;
+    case 1: pack_tile_condensed_body<T, 1>(p, data, stream, cols, lds_raw, dummies); break;
+    case 2: pack_tile_condensed_body<T, 2>(p, data, stream, cols, lds_raw, dummies); break;
+    case 3: pack_tile_condensed_body<T, 3>(p, data, stream, cols, lds_raw, dummies); break;
+    case 4: pack_tile_condensed_body<T, 4>(p, data, stream, cols, lds_raw, dummies); break;
+    default: pack_tile_condensed_body<T, -1>(p, data, stream, cols, lds_raw, dummies); break;
     }
 }
 
@@ -1096,10 +1211,11 @@ int64_t *noalloc_scratch()
 // and the lanes' dummy words.
 inline size_t pack_tile_lds(int64_t V, int w) { return (size_t)kWave * V * w + kWave * sizeof(int) + kWave * w; }
 
-// In-place pack of a table of rows of <= 64 elements in 64-row tiles (see
-// pack_tile_kernel); the column records live in a stream-ordered allocation.
+// Pack of a whole table of rows of <= 64 elements in 64-row tiles, in place
+// (pack_tile_kernel) or into a condensed stream (pack_tile_condensed_kernel);
+// the column records live in a stream-ordered allocation.
 template <typename T>
-int pack_tile_launch(const PackParams &p, hipStream_t stream)
+int pack_tile_launch(const PackParams &p, hipStream_t stream, bool condensed = false)
 {
     const DeviceInfo &dev = device_info();
     constexpr int W = (int)sizeof(T);
@@ -1120,6 +1236,9 @@ int pack_tile_launch(const PackParams &p, hipStream_t stream)
         tp.V = V;
         tp.max_runs = max_runs;
         tp.bitmask = p.bitmask;
+        tp.offsets = p.offsets;
+        tp.stream_bytes = p.stream_bytes;
+        tp.err = p.err;
         const size_t lds = pack_tile_lds(V, W);
         int64_t per_cu = (int64_t)(160 * 1024) / (int64_t)lds;
         if (per_cu > 8) per_cu = 8;
@@ -1127,8 +1246,12 @@ int pack_tile_launch(const PackParams &p, hipStream_t stream)
         const int64_t ntiles = (p.num_vecs + kWave - 1) / kWave;
         int64_t grid = (int64_t)dev.num_cus * per_cu;
         if (grid > ntiles) grid = ntiles;
-        hipLaunchKernelGGL((pack_tile_kernel<T>), dim3((unsigned)grid), dim3(kWave), lds, stream, tp, (T *)p.out,
-                           (const PackCol<T> *)cols);
+        if (condensed)
+            hipLaunchKernelGGL((pack_tile_condensed_kernel<T>), dim3((unsigned)grid), dim3(kWave), lds, stream, tp,
+                               (const T *)p.in, (T *)p.out, (const PackCol<T> *)cols);
+        else
+            hipLaunchKernelGGL((pack_tile_kernel<T>), dim3((unsigned)grid), dim3(kWave), lds, stream, tp, (T *)p.out,
+                               (const PackCol<T> *)cols);
         e = hipGetLastError();
     }
     IBP_HIP(hipFreeAsync(buf, stream));
@@ -1152,6 +1275,13 @@ int pack_typed(PackParams p, hipStream_t stream)
             p.n == p.num_vecs && p.n > 0 && p.V <= kWave && tile_rows_ok(p.V * (int64_t)sizeof(T)) &&
             pack_tile_lds(p.V, (int)sizeof(T)) * 6 <= 160 * 1024 && ((uintptr_t)p.in % 16) == 0)
             return pack_tile_launch<T>(p, stream);
+    }
+    if constexpr (sizeof(T) >= 4 && CONDENSED) {
+        // the condensed stream of a whole table with one mask, same tiles
+        if (p.index == nullptr && p.clusters == nullptr && p.n == p.num_vecs && p.n > 0 && p.V <= kWave &&
+            tile_rows_ok(p.V * (int64_t)sizeof(T)) && pack_tile_lds(p.V, (int)sizeof(T)) * 6 <= 160 * 1024 &&
+            ((uintptr_t)p.in % 16) == 0)
+            return pack_tile_launch<T>(p, stream, true);
     }
     if (ngroups < (int64_t)dev.num_cus * 8) {
         // Too few 32-row groups for a wave each: the waves of a block share a

@@ -118,6 +118,83 @@ def test_condensed_invalid_rows_are_zero_and_counted(ibp):
     assert not got[3].any() and torch.equal(got[2], torch.from_numpy(x[2]))
 
 
+def mixed_rows(oracle, n, v, w, seed, zero_frac=0.3):
+    """GNN-like rows (they pack under preprocess at 0.8) with about 40% of the
+    rows replaced by random bytes (they stay raw), and their mask/bitval."""
+    rng = np.random.default_rng(seed)
+    x = rows_of(n, v, w, seed=seed, zero_frac=zero_frac)
+    m, b, _, _ = oracle.preprocess(x, 0.8)
+    raw = rng.random(n) < 0.4
+    noise = rng.integers(0, 256, size=(n, v * w), dtype=np.uint8).view(x.dtype)
+    return np.ascontiguousarray(np.where(raw[:, None], noise, x)), m, b
+
+
+@pytest.mark.parametrize("n,v,w,zf", [(3000, 50, 8, 0.3), (1000, 64, 8, 0.3), (777, 48, 8, 0.3), (2000, 60, 4, 0.3),
+                                      (900, 64, 4, 0.3), (129, 16, 8, 0.9), (500, 32, 8, 0.9), (700, 36, 4, 0.9)])
+def test_condensed_whole_table_mixed_rows(ibp, oracle, n, v, w, zf):
+    """compress_condensed of a whole table with one mask (the row-per-lane
+    tile path for rows of whole 16-byte chunks): packed and raw rows
+    interleave, the stream and offsets equal the oracle's, device and pinned,
+    and the stream decompresses to the rows."""
+    x, m, b = mixed_rows(oracle, n, v, w, seed=n + v, zero_frac=zf)
+    ws, wo = oracle.compress_condensed(x, m, b)
+    sizes = np.diff(np.concatenate([[0], wo]))
+    assert (sizes == v * w).any() and (sizes < v * w).any()
+    mask, bitval = _dev(m.view(x.dtype)), _dev(b.view(x.dtype))
+    for where in ("device", "pinned"):
+        print("condensed whole table", n, v, w, where, flush=True)
+        src = _dev(x) if where == "device" else _pinned(x)
+        stream, off = ibp.compress_condensed(src, mask, bitval)
+        torch.cuda.synchronize()
+        assert np.array_equal(off.cpu().numpy(), wo), where
+        assert np.array_equal(stream.cpu().numpy().view(ws.dtype), ws), where
+        got = ibp.decompress(stream, off, mask, bitval, torch.device("cuda"), v)
+        torch.cuda.synchronize()
+        assert torch.equal(got.cpu(), torch.from_numpy(x)), where
+
+
+def test_condensed_pack_with_wrong_offsets_writes_only_consistent_rows(ibp, oracle):
+    """The pack half of compress_condensed through the C ABI with offsets whose
+    row lengths are not the rows' sizes: those rows are counted and left
+    unwritten, every other row is written exactly (whole-table tile path)."""
+    import ctypes
+
+    from invariantbitpacking_amd import _capi
+    n, v, w = 1000, 50, 8
+    x, m, b = mixed_rows(oracle, n, v, w, seed=77)
+    ws, wo = oracle.compress_condensed(x, m, b)
+    sizes = np.diff(np.concatenate([[0], wo]))
+    # rows 5 and 6 change lengths (row 5 grows by a word, row 6 shrinks), and
+    # one raw and one packed row elsewhere swap lengths with their neighbour
+    bad = wo.copy()
+    bad[5] += w
+    r_raw = int(np.flatnonzero(sizes[20:] == v * w)[0]) + 20
+    bad[r_raw - 1] += w  # row r_raw - 1 one word longer, row r_raw one word shorter
+    wrong = {5, 6, r_raw - 1, r_raw}
+    wrong = {i for i in wrong if (bad[i] - (bad[i - 1] if i else 0)) != sizes[i]}
+    xd = _dev(x)
+    mask, bitval = _dev(m.view(x.dtype)), _dev(b.view(x.dtype))
+    offs = torch.from_numpy(bad).cuda()
+    total = int(bad[-1])
+    stream = torch.full((total // w,), 0x5A5A5A5A5A5A5A5A, dtype=torch.int64, device="cuda")
+    ibp.index_errors(reset=True)
+    lay = _capi.Layout(offs.data_ptr(), total, None, 1)
+    _capi.call("ibp_compress_ex", w, xd.data_ptr(), stream.data_ptr(), n, v, mask.data_ptr(), bitval.data_ptr(),
+               None, 0, n, None, ctypes.byref(lay), ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+    torch.cuda.synchronize()
+    assert ibp.index_errors(reset=True) == len(wrong)
+    got = stream.cpu().numpy().view(np.uint8)
+    want = ws.view(np.uint8)
+    ends = np.concatenate([[0], wo])
+    bends = np.concatenate([[0], bad])
+    for i in range(n):
+        seg = got[bends[i]:bends[i + 1]]
+        if i in wrong:
+            assert (seg == 0x5A).all(), i
+        else:
+            assert np.array_equal(seg, want[ends[i]:ends[i + 1]]), i
+
+
 @pytest.mark.parametrize("w", [4, 8])
 @pytest.mark.parametrize("n,v", [(600, 50), (500, 64), (300, 130), (64, 1024)])
 def test_clusters_match_oracle(ibp, oracle, w, n, v):
