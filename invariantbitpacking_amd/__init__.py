@@ -25,11 +25,26 @@ import torch
 from ._capi import IBPError  # noqa: F401  (re-exported)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+class _Unbuilt:
+    """Stands in for the extension when it is not built, so that the package
+    (and its build script, invariantbitpacking_amd.csrc.build) can still be
+    imported; every call raises.  There is no fallback implementation."""
+
+    def __init__(self, err):
+        self._err = err
+
+    def __getattr__(self, name):
+        raise ImportError("the ibp_cuda extension (invariantbitpacking_amd/_C) is not built or cannot load "
+                          "libibp.so from %s; run `python -m invariantbitpacking_amd.csrc.build`: %s"
+                          % (_HERE, self._err))
+
+
 try:
     from . import _C
-except ImportError as e:  # no fallback: the library is the only implementation
-    raise ImportError("the ibp_cuda extension (invariantbitpacking_amd/_C) is not built or cannot load libibp.so "
-                      "from %s; run `python -m invariantbitpacking_amd.csrc.build`: %s" % (_HERE, e)) from e
+except ImportError as _e:
+    _C = _Unbuilt(_e)
 
 __all__ = ["print_debug", "preprocess", "preprocess_kmeans", "get_compress_size", "get_single_compress_len",
            "compress_inplace", "compress_condensed", "compress", "decompress_fetch", "decompress", "read_shared",

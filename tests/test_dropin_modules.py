@@ -123,3 +123,30 @@ def test_modules_forward_to_the_library():
             mod.get_compress_size(x, x[0], x[0], None, None)
     assert ibp.IBPError is not None
     assert "gfx950" in ibp.version()
+
+
+def test_unbuilt_package_imports_and_every_call_raises(tmp_path):
+    """A fresh checkout has no built extension: the package must still import
+    (its build script lives inside it), and every API call must raise
+    ImportError rather than fall back to anything else."""
+    import os
+    import shutil
+    import subprocess
+    import sys
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    pkg = os.path.join(root, "invariantbitpacking_amd")
+    dst = tmp_path / "invariantbitpacking_amd"
+    shutil.copytree(pkg, dst, ignore=shutil.ignore_patterns("*.so", "_build", "__pycache__", "*.o"))
+    code = ("import torch, invariantbitpacking_amd as m\n"
+            "from invariantbitpacking_amd.csrc import build\n"
+            "assert build.SOURCES\n"
+            "try:\n"
+            "    m.preprocess(torch.zeros(4, 4, dtype=torch.int64))\n"
+            "except ImportError as e:\n"
+            "    assert 'not built' in str(e), e\n"
+            "else:\n"
+            "    raise SystemExit('a call succeeded without the extension')\n")
+    r = subprocess.run([sys.executable, "-c", code], cwd=str(tmp_path), capture_output=True, text=True,
+                       env=dict(os.environ, PYTHONPATH=str(tmp_path)))
+    assert r.returncode == 0, r.stdout + r.stderr
