@@ -1285,10 +1285,12 @@ int pack_typed(PackParams p, hipStream_t stream)
     }
     if (ngroups < (int64_t)dev.num_cus * 8) {
         // Too few 32-row groups for a wave each: the waves of a block share a
-        // group (16 per block with slot buffers in device memory, as many as
-        // the LDS budget holds otherwise).
-        const bool lds = buf_bytes * 8 <= lds_budget;
-        int wpb = lds ? (int)(lds_budget / buf_bytes) : 16;
+        // group, with slot buffers in LDS when four or more fit one CU's LDS
+        // (as many waves as fit, up to 16), else 16 waves with buffers in
+        // device memory.
+        const int64_t lds_cu = 160 * 1024;
+        const bool lds = buf_bytes * 4 <= lds_cu;
+        int wpb = lds ? (int)((buf_bytes * 8 <= lds_budget ? lds_budget : lds_cu) / buf_bytes) : 16;
         if (wpb > 16) wpb = 16;
         p.split = wpb;
         const int64_t grid = ngroups > 0 ? ngroups : 1;
