@@ -366,12 +366,20 @@ __global__ __launch_bounds__(1024) void pack_kernel(PackParams p)
                     slot_fence<BUF_LDS>();
                 }
                 int64_t cursor = 0;
+                // chunk c0 + 64's loads are issued before chunk c0 is packed
+                T xn = lane < V ? src[lane] : T(0), mn = lane < V ? mask[lane] : T(0),
+                  bn = lane < V ? bitval[lane] : T(0);
                 for (int64_t c0 = 0; c0 < V; c0 += kWave) {
                     const int64_t j = c0 + lane;
                     const bool valid = j < V;
-                    const T x = valid ? src[j] : T(0);
-                    const T m = valid ? mask[j] : T(0);
-                    const bool f = valid && (T)(x & m) == (valid ? bitval[j] : T(0));
+                    const T x = xn, m = mn, bvj = bn;
+                    if (c0 + kWave < V) {
+                        const int64_t jn = j + kWave;
+                        xn = jn < V ? src[jn] : T(0);
+                        mn = jn < V ? mask[jn] : T(0);
+                        bn = jn < V ? bitval[jn] : T(0);
+                    }
+                    const bool f = valid && (T)(x & m) == bvj;
                     // Header flags of this chunk, LSB first (ibp_compress_dev.cuh:60-71):
                     // 64 / B header words, word k holding flags k*B .. k*B+B-1.
                     const uint64_t hb = ballot(f);
