@@ -333,6 +333,13 @@ int ibp_selftest_bitops(int trials, uint64_t seed);
  * release sequences on small arenas, each of which must drain with every
  * request's indices intact.  No GPU work. */
 int ibp_selftest_serve_admission(int trials, uint64_t seed);
+/* Runs the server's split copy of request indices into pinned memory (the
+ * submitting thread and helper threads claiming 64 KiB parts) on host
+ * buffers: `trials` copies of random sizes from each of three submitting
+ * threads at once, with idle gaps that put the helpers to sleep; every copy
+ * must match its source and leave the bytes around it untouched.  No GPU
+ * work. */
+int ibp_selftest_index_copy(int trials, uint64_t seed);
 
 #ifdef __cplusplus
 }

@@ -80,6 +80,7 @@ SIGNATURES = {
     "ibp_server_submit_at": (_I32, [_P, _I32, _P, _I64, ctypes.c_double, ctypes.POINTER(_I64)]),
     "ibp_server_clock_us": (ctypes.c_double, []),
     "ibp_selftest_serve_admission": (_I32, [_I32, ctypes.c_uint64]),
+    "ibp_selftest_index_copy": (_I32, [_I32, ctypes.c_uint64]),
     "ibp_server_stats_get": (_I32, [_P, ctypes.POINTER(ServerStats)]),
     "ibp_server_set_budget_override": (_I32, [_P, _SZ]),
     "ibp_server_request_times": (_I32, [_P, _I64, ctypes.POINTER(ctypes.c_double)]),

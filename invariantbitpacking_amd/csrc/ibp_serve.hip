@@ -12,6 +12,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -23,6 +24,7 @@
 #include <vector>
 
 #include "ibp_batch.h"
+#include "ibp_index_copy.h"
 #include "ibp_internal.h"
 
 using namespace ibp;
@@ -235,6 +237,7 @@ struct ibp_server {
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::thread worker;
+    IndexCopier copier;
     bool stop = false;
     int64_t next_id = 1;
     int64_t completed = 0, launches = 0, rows = 0, launched_reqs = 0;
@@ -500,7 +503,7 @@ extern "C" int ibp_server_submit_at(ibp_server *srv, int set_id, const int64_t *
     lk.unlock();
     // The copy runs outside the lock; the request is not queued until it is done.
     std::vector<int64_t> vec;
-    if (pinned) memcpy(srv->pin_mem + ioff, index, sizeof(int64_t) * (size_t)n);
+    if (pinned) srv->copier.copy(srv->pin_mem + ioff, (const char *)index, sizeof(int64_t) * (size_t)n);
     else vec.assign(index, index + n);
     lk.lock();
     Req &r = srv->adm.reqs[id];
@@ -781,4 +784,46 @@ extern "C" int ibp_selftest_serve_admission(int trials, uint64_t seed)
     std::mt19937_64 rng(seed);
     for (int t = 0; st == IBP_OK && t < trials; ++t) st = admission_random_case(rng, 200);
     return st;
+}
+
+// Host self-test of IndexCopier: copies of random sizes (below, at and far
+// above the split threshold, with odd tails) from several submitting threads
+// at once, in bursts and after idle gaps long enough for the helpers to fall
+// asleep; every destination byte is checked, and the bytes around it must be
+// untouched.
+extern "C" int ibp_selftest_index_copy(int trials, uint64_t seed)
+{
+    clear_error();
+    IndexCopier cp;
+    std::atomic<int> bad{0};
+    auto worker = [&](uint64_t s) {
+        std::mt19937_64 rng(s);
+        std::vector<unsigned char> src, dst;
+        for (int t = 0; t < trials && bad.load() == 0; ++t) {
+            const size_t sizes[] = {1, 4096, IndexCopier::kMinSplit - 8, IndexCopier::kMinSplit,
+                                    IndexCopier::kMinSplit + 8, (size_t)2 << 20};
+            size_t n = sizes[rng() % 6];
+            if (rng() % 2) n = 1 + rng() % ((size_t)3 << 20);
+            src.resize(n);
+            for (size_t i = 0; i < n; ++i) src[i] = (unsigned char)(rng() >> 56);
+            dst.assign(n + 128, 0xA5);
+            cp.copy((char *)dst.data() + 64, (const char *)src.data(), n);
+            for (size_t i = 0; i < 64; ++i)
+                if (dst[i] != 0xA5 || dst[64 + n + i] != 0xA5) {
+                    bad.store(1);
+                    return;
+                }
+            if (memcmp(dst.data() + 64, src.data(), n) != 0) {
+                bad.store(2);
+                return;
+            }
+            if (rng() % 16 == 0) std::this_thread::sleep_for(std::chrono::microseconds(300 + rng() % 2000));
+        }
+    };
+    std::vector<std::thread> th;
+    for (int k = 0; k < 3; ++k) th.emplace_back(worker, seed + (uint64_t)k * 7919);
+    for (std::thread &t : th) t.join();
+    if (bad.load()) return set_error(IBP_ERR_INVALID, "index copy self-test: %s",
+                                     bad.load() == 1 ? "bytes outside the destination changed" : "copy differs");
+    return IBP_OK;
 }

@@ -49,6 +49,13 @@ def test_selftest_serve_admission_host(capi):
     assert capi.lib().ibp_selftest_serve_admission(300, 7) == 0, capi.last_error()
 
 
+def test_selftest_index_copy_host(capi):
+    """The server's split copy of request indices (the submitter and three
+    helpers claiming 64 KiB parts), from three submitting threads at once,
+    with idle gaps that put the helpers to sleep: every copy exact."""
+    assert capi.lib().ibp_selftest_index_copy(150, 11) == 0, capi.last_error()
+
+
 def test_invalid_arguments_are_status_codes(capi):
     L = capi.lib()
     st = L.ibp_decompress_fetch(3, None, None, 10, 4, None, None, None, None, 0, 10, 0, 0, 0, -1, 0, None)
