@@ -1151,7 +1151,7 @@ def main():
     ap.add_argument("--workload", default="gnn_minibatch",
                     choices=["gnn_minibatch", "gnn_features", "dlrm", "serve", "llm", "pcie_rows", "kernels"],
                     help="gnn_minibatch is the headline line; the others print their own JSON")
-    ap.add_argument("--rates", type=float, nargs="+", default=[1e2, 1e3, 1e4])
+    ap.add_argument("--rates", type=float, nargs="+", default=[1e2, 3e2, 1e3, 3e3, 1e4])
     ap.add_argument("--layers", type=int, default=32, help="decoder layers of the llm workload")
     ap.add_argument("--ops", default="", help="kernels workload: comma-separated subset of its ops")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter passes of the default run")
