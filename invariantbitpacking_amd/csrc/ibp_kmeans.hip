@@ -569,7 +569,7 @@ extern "C" int ibp_preprocess_kmeans(int elem_bytes, const void *data, int64_t n
     IBP_CHECK(vec_size > 0 && num_vecs >= 0, "invalid shape");
     IBP_CHECK(num_clusters >= 1 && num_clusters <= 65535, "num_clusters must be in [1, 65535]");
     IBP_CHECK(num_vecs < (int64_t)1 << 31, "num_vecs must fit int32 cluster ids and counters");
-    IBP_CHECK(masks && bitvals && clusters && (data || num_vecs == 0), "null pointer argument");
+    IBP_CHECK(masks && bitvals && ((data && clusters) || num_vecs == 0), "null pointer argument");
     const int st = elem_bytes == 8
                        ? kmeans_typed<uint64_t>(data, num_vecs, vec_size, num_clusters, seed, masks, bitvals, clusters,
                                                 as_stream(stream))

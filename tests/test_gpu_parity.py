@@ -400,6 +400,38 @@ def test_fetch_edge_cases(ibp, oracle):
     assert tuple(out1.shape) == (1, 64) and np.array_equal(_np(out1)[0], data[7])
 
 
+@pytest.mark.parametrize("w", [4, 8])
+def test_empty_tables(ibp, oracle, w):
+    """A table of zero rows (on the GPU: an empty host tensor is not pinned,
+    so the calls reject it, as the reference's checks do) through every call:
+    zero masks (nothing to save), empty sizes, bitmask, stream and offsets,
+    fetches of zero rows; k-means gives the oracle's all-zero centroids and no
+    clusters."""
+    v = 8
+    x = np.zeros((0, v), dtype=np.dtype("i%d" % w))
+    om, ob, _, _ = oracle.preprocess(x, 0.9)
+    okm, okb, okc = oracle.kmeans(x, 4, seed=1)
+    for where in ("device",):
+        print("empty table", w, where, flush=True)
+        t = place(x, where)
+        for thr in (0.9, None):
+            m, b = ibp.preprocess(t, thr)
+            assert np.array_equal(_u(_np(m)), om) and np.array_equal(_u(_np(b)), ob)
+        m, b = ibp.preprocess(place(np.ones((3, v), dtype=x.dtype), "device"), 0.9)
+        sz = ibp.get_compress_size(t, m, b)
+        assert tuple(sz.shape) == (0,)
+        bm = ibp.compress_inplace(t, m, b)
+        assert tuple(bm.shape) == (0,)
+        stream, off = ibp.compress_condensed(t, m, b)
+        assert stream.numel() == 0 and off.numel() == 0
+        out = ibp.decompress_fetch(t, m, b, bm, torch.device("cuda"))
+        assert tuple(out.shape) == (0, v)
+        back = ibp.decompress(stream, off, m, b, torch.device("cuda"), v)
+        assert tuple(back.shape) == (0, v)
+        km, kb, kc = ibp.preprocess_kmeans(t, 4, 0.85, seed=1)
+        assert np.array_equal(_u(_np(km)), okm) and np.array_equal(_u(_np(kb)), okb) and kc.numel() == 0
+
+
 def test_fetch_full_width_fields(ibp, oracle):
     """Raw 64-bit elements at 64-aligned stream positions and flagged elements
     with mask 0 (the full-width shift cases) through compress + fetch."""
