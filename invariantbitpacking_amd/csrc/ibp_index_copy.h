@@ -110,7 +110,10 @@ class IndexCopier {
             }
             std::unique_lock<std::mutex> lk(m_);
             sleepers_.fetch_add(1, std::memory_order_acq_rel);
-            cv_.wait_for(lk, std::chrono::milliseconds(5), [&] {
+            // a system_clock deadline: ThreadSanitizer follows this wait (it
+            // does not follow the steady_clock one); the predicate is checked
+            // on every wake-up, so a clock step only changes a sleep's length
+            cv_.wait_until(lk, std::chrono::system_clock::now() + std::chrono::milliseconds(5), [&] {
                 return stop_.load(std::memory_order_acquire) ||
                        (claim_.load(std::memory_order_acquire) >> 40) != seen;
             });

@@ -91,6 +91,24 @@ bool host_temp(const OptTensor &orig, const Tensor &used)
 
 void *cur_stream() { return c10::hip::getCurrentHIPStream().stream(); }
 
+// Makes the GPU that holds `t` (or `d`) current for the rest of the call, so
+// outputs, workspaces, the stream and the kernels are that GPU's; host-resident
+// (pinned) data leaves the current GPU in place.
+struct CallDevice {
+    c10::OptionalDeviceGuard g;
+    explicit CallDevice(const Tensor &t)
+    {
+        if (t.is_cuda()) g.reset_device(t.device());
+    }
+    CallDevice(const c10::Device &d, const Tensor &t)
+    {
+        if (d.is_cuda() && d.has_index())
+            g.reset_device(d);
+        else if (t.is_cuda())
+            g.reset_device(t.device());
+    }
+};
+
 // Waits for the current stream with the GIL released (other Python threads
 // keep running, as they do while a ctypes call waits).
 void sync_stream()
@@ -249,6 +267,7 @@ std::tuple<Tensor, Tensor> preprocess(const Tensor &dataset, std::optional<doubl
     const int64_t w = dataset.element_size();
     need(width_ok(w), "Input tensor must be 1, 2, 4, or 8-byte datatype");
     need(dataset.is_contiguous(), "Input tensor must be contiguous");
+    CallDevice dg(dataset);
     const float t = threshold.has_value() ? (float)*threshold : -1.0f;
     const int64_t n = dataset.size(0), v = dataset.size(1);
     Tensor mask = at::zeros({v}, on_gpu(signed_type(w)));
@@ -272,6 +291,7 @@ std::tuple<Tensor, Tensor, Tensor> preprocess_kmeans(const Tensor &dataset, int6
     const int64_t w = dataset.element_size();
     need(w == 4 || w == 8, "Input tensor must be 4-byte or 8-byte datatype");
     need(dataset.is_contiguous(), "Input tensor must be contiguous");
+    CallDevice dg(dataset);
     const int64_t n = dataset.size(0), v = dataset.size(1);
     Tensor masks = at::zeros({num_clusters, v}, on_gpu(signed_type(w)));
     Tensor bitvals = at::zeros({num_clusters, v}, on_gpu(signed_type(w)));
@@ -299,6 +319,7 @@ Tensor get_compress_size(const Tensor &dataset, const Tensor &mask, const Tensor
     const int64_t w = dataset.element_size();
     auto [num_vecs, v] = rows_cols(dataset);
     mask_checks(dataset, mask, bitval, v);
+    CallDevice dg(dataset);
     Index ix = index_arg(index, false);
     const int64_t n = ix.t.defined() ? ix.t.size(0) : num_vecs;
     Tensor total;
@@ -334,6 +355,7 @@ Tensor compress_inplace(const Tensor &dataset, const Tensor &mask, const Tensor 
     const int64_t w = dataset.element_size();
     auto [num_vecs, v] = rows_cols(dataset);
     mask_checks(dataset, mask, bitval, v);
+    CallDevice dg(dataset);
     Index ix = index_arg(index, false);
     Clusters cl = clusters_arg(clusters, mask, num_vecs);
     const int64_t n = ix.t.defined() ? ix.t.size(0) : num_vecs;
@@ -368,6 +390,7 @@ std::tuple<Tensor, Tensor> compress_condensed(const Tensor &dataset, const Tenso
     const int64_t w = dataset.element_size();
     auto [num_vecs, v] = rows_cols(dataset);
     mask_checks(dataset, mask, bitval, v);
+    CallDevice dg(dataset);
     Index ix = index_arg(index, false);
     Clusters cl = clusters_arg(clusters, mask, num_vecs);
     const int64_t n = ix.t.defined() ? ix.t.size(0) : num_vecs;
@@ -411,6 +434,7 @@ Tensor decompress_fetch(const Tensor &comp, const Tensor &mask, const Tensor &bi
     need(width_ok(w), "Input tensor must be 1, 2, 4 or 8-byte datatype");
     auto [num_vecs, v] = rows_cols(comp);
     mask_checks(comp, mask, bitval, v);
+    CallDevice dg(out_device, comp);
     need(bitmask.is_cuda() || accessible(bitmask), "Bitmask tensor must accessible by CUDA device");
     need(bitmask.element_size() == 4 && bitmask.is_contiguous(), "Bitmask must be a contiguous int32 tensor");
     need(bitmask.numel() >= (num_vecs + 31) / 32, "Bitmask too short for the dataset");
@@ -444,6 +468,7 @@ Tensor decompress(const Tensor &stream, const Tensor &offsets, const Tensor &mas
     need(width_ok(w), "Input tensor must be 1, 2, 4 or 8-byte datatype");
     need(vec_size > 0, "vec_size must be positive");
     mask_checks(stream, mask, bitval, vec_size);
+    CallDevice dg(device, stream);
     const int64_t num_rows = offsets.numel();
     Index ix = index_arg(index, true);
     const int64_t n = ix.t.defined() ? ix.t.size(0) : num_rows;

@@ -453,3 +453,26 @@ def test_kmeans_or_chain_known_answer(oracle):
     em, eb, ecl = or_chain_expected()
     assert np.array_equal(cl, ecl)
     assert np.array_equal(m.view(np.uint32), em) and np.array_equal(b.view(np.uint32), eb)
+
+
+def test_oracle_under_address_and_undefined_behaviour_sanitizers(tmp_path):
+    """The oracle, built with AddressSanitizer and UndefinedBehaviorSanitizer
+    (host code), run over random tables of every width and shape class by a C
+    driver that also checks its round trips: no report, every case passes."""
+    import os
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("no gcc")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = tmp_path / "oracle_san"
+    r = subprocess.run(["gcc", "-O1", "-g", "-std=c11", "-fsanitize=address,undefined", "-fno-sanitize-recover=all",
+                        "-fno-omit-frame-pointer", "-fopenmp", "-I" + os.path.join(root, "oracle"),
+                        os.path.join(root, "oracle", "ibp_oracle.c"),
+                        os.path.join(root, "tests", "oracle_sanitize_driver.c"), "-o", str(exe)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    for seed in (1, 2):
+        r = subprocess.run([str(exe), str(seed), "120"], capture_output=True, text=True, timeout=240,
+                           env=dict(os.environ, ASAN_OPTIONS="detect_leaks=1"))
+        assert r.returncode == 0 and "passed" in r.stdout, r.stdout + r.stderr
