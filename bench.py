@@ -457,7 +457,7 @@ def run_gnn_minibatch(args, traffic=None):
         ibp.decompress_fetch(comp, mask, bitval, bitmask, torch.device("cuda"), comp_len, idx[k], None, None, None,
                              out)
         exact += bool(torch.equal(out, x[idx[k]]))
-    lat = call_latencies(fetch, max(20, args.steps))
+    lat = call_latencies(fetch, max(200, args.steps))
     # side figure: a separate output per minibatch (5.1 GB of outputs), so the
     # rows written are not the ones the previous call left in the caches
     outs = [torch.empty((batch, v), dtype=torch.int64, device="cuda") for _ in range(nsets)]
@@ -517,7 +517,7 @@ def run_gnn_minibatch(args, traffic=None):
                              "decompressed_gbps": round(args.steps * dec_bytes / rot_wall / 1e9, 2),
                              "kernel_ms": round(rot_ms, 5), "bitexact": rot_exact == len(timed)},
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4),
-                       "what": "isolated call: decompress_fetch issued -> rows ready (host wall clock)"},
+                       "what": "isolated call: decompress_fetch issued -> rows ready (host wall clock), %d calls" % len(lat)},
         "batched": batched,
         "pcie_resident": pcie,
         "compression": {"gpu": gpu_compress, "cpu": cpu_compress},
@@ -847,7 +847,7 @@ def run_gnn_features(args, calls=64):
             k = step[0] % calls
             ibp.decompress_fetch(src, mask, bitval, bm, dev, cl, idx[k], None, None, None, outs[k])
             step[0] += 1
-        lat = call_latencies(one, max(50, args.steps))
+        lat = call_latencies(one, max(1000, args.steps))
         p_ms, _ = timed_region(lambda: [one() for _ in range(calls)], max(3, args.steps // 4), 1)
         p_exact = all(bool(torch.equal(o, w)) for o, w in zip(outs, want))
         # host cost of one call: issue `calls` calls without waiting for the GPU
@@ -869,6 +869,7 @@ def run_gnn_features(args, calls=64):
         b_exact = all(bool(torch.equal(o, w)) for o, w in zip(outs, want))
         bf.close()
         res[where] = {"p50_ms": round(float(np.percentile(lat, 50)), 4), "p99_ms": round(float(np.percentile(lat, 99)), 4),
+                      "latency_calls": len(lat),
                       "plain_ms_per_call": round(p_ms / calls, 5),
                       "plain_gbps": round(calls * call_bytes / (p_ms / 1e3) / 1e9, 1),
                       "host_us_per_call": round(host_us, 2),

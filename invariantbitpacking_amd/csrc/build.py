@@ -109,7 +109,8 @@ def build(jobs=None, verbose=False):
             if err.strip():
                 sys.stderr.write(err)
     tmp = OUT + ".tmp"
-    cmd = [hipcc(), "-shared", "-o", tmp] + [o for o, _ in results] + ["--offload-arch=" + ARCH, "-fPIC"]
+    cmd = [hipcc(), "-shared", "-o", tmp] + [o for o, _ in results] + [
+        "--offload-arch=" + ARCH, "-fPIC", "-L" + os.path.join(rocm_path(), "lib"), "-lrocprofiler-sdk-roctx"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n%s%s" % (r.stdout, r.stderr))

@@ -446,12 +446,14 @@ extern "C" int ibp_plan_destroy(ibp_plan *pl)
 
 extern "C" int ibp_plan_launch(ibp_plan *pl, void *stream)
 {
+    IBP_TRACE("ibp_plan_launch");
     IBP_CHECK(pl != nullptr, "null plan");
     return launch(pl, as_stream(stream));
 }
 
 extern "C" int ibp_plan_autotune(ibp_plan *pl, void *stream, float *best_ms)
 {
+    IBP_TRACE("ibp_plan_autotune");
     IBP_CHECK(pl != nullptr, "null plan");
     hipStream_t s = as_stream(stream);
     const DeviceInfo &dev = device_info();
@@ -521,6 +523,7 @@ extern "C" int ibp_plan_capture_graph(ibp_plan *pl, void *stream)
 
 extern "C" int ibp_plan_graph_launch(ibp_plan *pl, void *stream)
 {
+    IBP_TRACE("ibp_plan_graph_launch");
     IBP_CHECK(pl != nullptr, "null plan");
     if (!pl->graph_exec) {
         const int st = ibp_plan_capture_graph(pl, stream);

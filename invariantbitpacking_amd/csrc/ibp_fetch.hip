@@ -101,6 +101,7 @@ extern "C" int ibp_decompress_fetch_ex(int elem_bytes, void *out, const void *co
                                        int comp_len, int nblks, int nthreads, int impl, int comp_loc,
                                        const ibp_layout *layout, void *stream)
 {
+    IBP_TRACE("ibp_decompress_fetch_ex");
     IBP_CHECK(elem_bytes == 1 || elem_bytes == 2 || elem_bytes == 4 || elem_bytes == 8,
               "Input tensor must be 1, 2, 4 or 8-byte datatype (got %d)", elem_bytes);
     IBP_CHECK(vec_size > 0 && num_vecs >= 0 && n >= 0, "invalid shape num_vecs=%lld vec_size=%lld n=%lld",

@@ -1,6 +1,7 @@
 // ibp_internal.h -- host-side helpers shared by the libibp translation units.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 #include <stdarg.h>
 #include <stdint.h>
 
@@ -34,6 +35,16 @@ bool pointer_is_host(const void *p);
 
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
+// A named range around one C ABI call, seen by `rocprofv3 --marker-trace`
+// beside the kernels the call launches (without a profiler attached the
+// markers cost a few tens of nanoseconds).
+struct TraceRange {
+    explicit TraceRange(const char *name) { roctxRangePushA(name); }
+    ~TraceRange() { roctxRangePop(); }
+    TraceRange(const TraceRange &) = delete;
+    TraceRange &operator=(const TraceRange &) = delete;
+};
+
 // Words of the speculative first read of a packed row of <= 64 elements.
 int first_words_for(int64_t vec_size, int elem_bytes, int comp_len, bool host_resident);
 
@@ -54,3 +65,6 @@ int first_words_for(int64_t vec_size, int elem_bytes, int comp_len, bool host_re
 
 // After a kernel launch.
 #define IBP_LAUNCH_CHECK() IBP_HIP(hipGetLastError())
+
+// The marker range of the entry point it opens.
+#define IBP_TRACE(name) ::ibp::TraceRange ibp_trace_range_(name)

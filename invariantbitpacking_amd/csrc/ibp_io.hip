@@ -22,6 +22,7 @@ using namespace ibp;
 
 extern "C" int ibp_read_file(const char *path, void *dst, size_t capacity, int threads, size_t *bytes_read)
 {
+    IBP_TRACE("ibp_read_file");
     IBP_CHECK(path != nullptr, "null path");
     if (bytes_read) *bytes_read = 0;
     const int fd = open(path, O_RDONLY | O_CLOEXEC);

@@ -564,6 +564,7 @@ extern "C" int ibp_preprocess_kmeans(int elem_bytes, const void *data, int64_t n
                                      int num_clusters, float threshold, int64_t seed, void *masks, void *bitvals,
                                      int32_t *clusters, void *stream)
 {
+    IBP_TRACE("ibp_preprocess_kmeans");
     (void)threshold; // unused by the reference as well (ibp_preproc_host.cuh:119-284)
     IBP_CHECK(elem_bytes == 4 || elem_bytes == 8, "Input tensor must be 4-byte or 8-byte datatype (got %d)", elem_bytes);
     IBP_CHECK(vec_size > 0 && num_vecs >= 0, "invalid shape");

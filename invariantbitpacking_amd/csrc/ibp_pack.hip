@@ -1387,6 +1387,7 @@ extern "C" int ibp_get_compress_size_ex(int elem_bytes, const void *data, int64_
                                         int64_t n, int64_t *sizes, uint64_t *total, const ibp_layout *layout,
                                         void *stream)
 {
+    IBP_TRACE("ibp_get_compress_size_ex");
     IBP_CHECK_WIDTH(elem_bytes);
     IBP_CHECK(vec_size > 0 && num_vecs >= 0 && n >= 0, "invalid shape");
     IBP_CHECK(index == nullptr || index_bytes == 4 || index_bytes == 8, "index must be int32 or int64");
@@ -1413,6 +1414,7 @@ extern "C" int ibp_condensed_offsets(int elem_bytes, const void *data, int64_t n
                                      const void *mask, const void *bitval, const void *index, int index_bytes,
                                      int64_t n, int64_t *offsets, const ibp_layout *layout, void *stream)
 {
+    IBP_TRACE("ibp_condensed_offsets");
     IBP_CHECK_WIDTH(elem_bytes);
     IBP_CHECK(vec_size > 0 && num_vecs >= 0 && n >= 0, "invalid shape");
     IBP_CHECK(index == nullptr || index_bytes == 4 || index_bytes == 8, "index must be int32 or int64");
@@ -1444,6 +1446,7 @@ extern "C" int ibp_condensed_offsets_noalloc(int elem_bytes, const void *data, i
                                              const void *index, int index_bytes, int64_t n, int64_t *offsets,
                                              const ibp_layout *layout, void *stream)
 {
+    IBP_TRACE("ibp_condensed_offsets_noalloc");
     IBP_CHECK_WIDTH(elem_bytes);
     IBP_CHECK(vec_size > 0 && num_vecs >= 0 && n >= 0, "invalid shape");
     IBP_CHECK(index == nullptr || index_bytes == 4 || index_bytes == 8, "index must be int32 or int64");
@@ -1474,6 +1477,7 @@ extern "C" int ibp_compress_ex(int elem_bytes, const void *input, void *output, 
                                const void *mask, const void *bitval, const void *index, int index_bytes, int64_t n,
                                int32_t *bitmask, const ibp_layout *layout, void *stream)
 {
+    IBP_TRACE("ibp_compress_ex");
     IBP_CHECK_WIDTH(elem_bytes);
     IBP_CHECK(vec_size > 0 && num_vecs >= 0 && n >= 0, "invalid shape");
     IBP_CHECK(index == nullptr || index_bytes == 4 || index_bytes == 8, "index must be int32 or int64");

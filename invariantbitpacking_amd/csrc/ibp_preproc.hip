@@ -837,6 +837,7 @@ extern "C" int ibp_preprocess(int elem_bytes, const void *data, int64_t num_vecs
                               void *mask, void *bitval, int32_t *chosen_step, void *workspace,
                               size_t workspace_bytes, void *stream)
 {
+    IBP_TRACE("ibp_preprocess");
     IBP_CHECK(elem_bytes == 1 || elem_bytes == 2 || elem_bytes == 4 || elem_bytes == 8,
               "Input tensor must be 1, 2, 4, or 8-byte datatype (got %d)", elem_bytes);
     IBP_CHECK(vec_size > 0 && num_vecs >= 0, "invalid shape");

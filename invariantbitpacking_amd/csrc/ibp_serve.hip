@@ -484,6 +484,7 @@ extern "C" double ibp_server_clock_us(void) { return now_us(); }
 extern "C" int ibp_server_submit_at(ibp_server *srv, int set_id, const int64_t *index, int64_t n, double arrival_us,
                                     int64_t *req_id)
 {
+    IBP_TRACE("ibp_server_submit_at");
     const double t_call = now_us(); // latency includes copying the index
     IBP_CHECK(srv && req_id, "null argument");
     IBP_CHECK(n >= 1 && index != nullptr, "a request needs at least one index");
@@ -528,6 +529,7 @@ extern "C" int ibp_server_submit(ibp_server *srv, int set_id, const int64_t *ind
 
 extern "C" int ibp_server_wait(ibp_server *srv, int64_t req_id, int timeout_ms, void **out, double *latency_us)
 {
+    IBP_TRACE("ibp_server_wait");
     IBP_CHECK(srv, "null server");
     std::unique_lock<std::mutex> lk(srv->mu);
     IBP_CHECK(srv->adm.reqs.count(req_id) != 0, "unknown request %lld", (long long)req_id);
