@@ -357,9 +357,11 @@ __global__ __launch_bounds__(256) void score_rows_kernel(const T *__restrict__ d
 // each: a tile holds 64 rows whatever the number of waves scoring it, so
 // four waves per tile give four times the waves per CU that one wave per
 // tile would, and each does a quarter of the vector work (GNN table, 7
-// thresholds: 237 us at two per wave, 280 at four, 309 at seven).  With DB
-// the block keeps two tile buffers and issues the next tile's copy before
-// scoring the current one; that halves the blocks per CU and is slower.
+// thresholds: 237 us at two per wave, 280 at four, 309 at seven).  The next
+// tile is loaded into registers (about 7 chunks per lane) while the current
+// one is scored, and stored to LDS after it: one LDS tile per block, and the
+// loads overlap the scoring (a second LDS tile filled with copy_tile did not
+// overlap: the compiler waits for every pending LDS copy before any LDS read).
 // Used where the row stride in dwords is not a multiple of 16 modulo 32, so
 // eight lanes reading 16 bytes each hit at most two bank groups twice.
 
@@ -446,11 +448,13 @@ __device__ __forceinline__ void need_rec(const T (&m)[ScoreRec<T, TPW, NB>::EPC]
     }
 }
 
-// TPW thresholds per wave (t0 = wave * TPW), WPT waves per tile; DB: two
-// tile buffers per block, the next tile's copy issued before the current
-// one is scored.  Every threshold of a wave is scored without a branch (a
-// zero mask loses nothing), with the mask popcounts from the records.
-template <typename T, int TPW, int WPT, bool DB, bool NESTED>
+// TPW thresholds per wave (t0 = wave * TPW), WPT waves per tile; RP: the
+// next tile is loaded into registers while the current one is scored from
+// LDS, then stored to LDS (otherwise each tile is copied to LDS directly and
+// waited for before it is scored).  Every threshold of a wave is scored
+// without a branch (a zero mask loses nothing), with the mask popcounts from
+// the records.
+template <typename T, int TPW, int WPT, bool RP, bool NESTED>
 __global__ __launch_bounds__(64 * WPT) void score_tile_kernel(const T *__restrict__ data, int64_t num_vecs, int V,
                                                               const ScoreRec<T, TPW, NESTED ? 1 : TPW> *__restrict__ recs,
                                                               const uint32_t *__restrict__ colp, int nt,
@@ -463,7 +467,6 @@ __global__ __launch_bounds__(64 * WPT) void score_tile_kernel(const T *__restric
     const int wib = wave_in_block_id();
     const int t0 = wib * TPW; // this wave's first threshold
     const int C = V * (int)sizeof(T) / 16;
-    const size_t tile_bytes = (size_t)C * kWave * 16;
     const int64_t ntiles = (num_vecs + kWave - 1) / kWave;
     const int64_t nblocks = gridDim.x;
     const bool swz = tile_swizzled((int64_t)C * 16);
@@ -483,18 +486,22 @@ __global__ __launch_bounds__(64 * WPT) void score_tile_kernel(const T *__restric
 #pragma unroll
     for (int t = 0; t < TPW; ++t) pcsum[t] = (uint32_t)wave_sum((int)pcsum[t]);
     const bool active = t0 < nt; // waves past the last threshold only copy
-    int cur = 0;
+    constexpr int NPRE = RP ? (kTileMaxRowBytes / 16 + WPT - 1) / WPT : 1;
+    tile_chunk_t pre[NPRE];
     int64_t tl = blockIdx.x;
-    if (DB && tl < ntiles) copy_tile<T, WPT>(data, num_vecs, V, C, tl, wib, lane, tiles, swz);
+    if (RP && tl < ntiles) load_tile_regs<T, WPT, NPRE>(data, num_vecs, V, C, tl, wib, lane, swz, pre);
     for (; tl < ntiles; tl += nblocks) {
-        if (!DB) copy_tile<T, WPT>(data, num_vecs, V, C, tl, wib, lane, tiles, swz);
-        __builtin_amdgcn_s_waitcnt(0x0f70); // vmcnt(0): this wave's part of the tile has landed
-        __syncthreads();                    // ... and every other wave's part
-        const unsigned char *buf = tiles + (size_t)cur * tile_bytes;
-        // the next tile goes to the other buffer, read by the previous
-        // iteration (finished at the barrier above)
-        if (DB && tl + nblocks < ntiles)
-            copy_tile<T, WPT>(data, num_vecs, V, C, tl + nblocks, wib, lane, tiles + (size_t)(cur ^ 1) * tile_bytes, swz);
+        if (RP) {
+            store_tile_regs<WPT, NPRE>(tiles, C, wib, lane, pre);
+            __syncthreads(); // every wave's part of the tile is in LDS
+            if (tl + nblocks < ntiles)
+                load_tile_regs<T, WPT, NPRE>(data, num_vecs, V, C, tl + nblocks, wib, lane, swz, pre);
+        } else {
+            copy_tile<T, WPT>(data, num_vecs, V, C, tl, wib, lane, tiles, swz);
+            __builtin_amdgcn_s_waitcnt(0x0f70); // vmcnt(0): this wave's part of the tile has landed
+            __syncthreads();                    // ... and every other wave's part
+        }
+        const unsigned char *buf = tiles;
         if (active) {
             const uint4 *mine = reinterpret_cast<const uint4 *>(buf) + lane * C;
             const int64_t row = tl * kWave + lane;
@@ -550,11 +557,7 @@ __global__ __launch_bounds__(64 * WPT) void score_tile_kernel(const T *__restric
                 }
             }
         }
-        if (DB) {
-            cur ^= 1;
-        } else {
-            __syncthreads(); // this tile's reads are done before the next copy lands
-        }
+        __syncthreads(); // this tile's reads are done before the next one is stored
     }
     // One atomic per threshold and wave: the lanes' savings are summed as
     // 24-bit low parts and the rest (each sum of 64 fits 32 bits as long as a
@@ -575,8 +578,8 @@ __global__ __launch_bounds__(64 * WPT) void score_tile_kernel(const T *__restric
 #ifndef IBP_SCORE_SWEEP_TPW
 #define IBP_SCORE_SWEEP_TPW 2
 #endif
-#ifndef IBP_SCORE_DB
-#define IBP_SCORE_DB 0
+#ifndef IBP_SCORE_RP
+#define IBP_SCORE_RP 1
 #endif
 constexpr int kScoreTPW = IBP_SCORE_SWEEP_TPW;
 constexpr int kScoreWPT = (kMaxThresholds - 1 + kScoreTPW - 1) / kScoreTPW;
@@ -587,13 +590,13 @@ void launch_score_tile_cfg(const T *data, int64_t N, int64_t V, const T *colm, c
                            hipStream_t stream)
 {
     constexpr int NB = NESTED ? 1 : TPW;
-    constexpr bool DB = IBP_SCORE_DB != 0;
+    constexpr bool RP = IBP_SCORE_RP != 0;
     using Rec = ScoreRec<T, TPW, NB>;
     const int C = (int)(V * (int64_t)sizeof(T) / 16);
     const int64_t threads = (int64_t)WPT * (C + 1) * Rec::EPC * TPW;
     hipLaunchKernelGGL((score_recs_kernel<T, TPW, NB>), dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, stream,
                        colm, colb, colp, V, C, WPT, (Rec *)recbuf);
-    hipLaunchKernelGGL((score_tile_kernel<T, TPW, WPT, DB, NESTED>), dim3((unsigned)grid), dim3(kWave * WPT), lds,
+    hipLaunchKernelGGL((score_tile_kernel<T, TPW, WPT, RP, NESTED>), dim3((unsigned)grid), dim3(kWave * WPT), lds,
                        stream, data, N, (int)V, (const Rec *)recbuf, colp, nt, scores);
 }
 
@@ -616,8 +619,7 @@ template <typename T>
 void launch_score_tile(const T *data, int64_t N, int64_t V, const T *colm, const T *colb, const uint32_t *colp,
                        int nt, bool nested, unsigned long long *scores, void *recbuf, int num_cus, hipStream_t stream)
 {
-    constexpr bool DB = IBP_SCORE_DB != 0;
-    const size_t lds = (DB ? 2 : 1) * (size_t)kWave * (size_t)V * sizeof(T);
+    const size_t lds = (size_t)kWave * (size_t)V * sizeof(T);
     const int64_t ntiles = (N + kWave - 1) / kWave;
     int64_t per_cu = (int64_t)(160 * 1024) / (int64_t)lds;
     if (per_cu > 8) per_cu = 8;

@@ -50,6 +50,44 @@ __device__ __forceinline__ void copy_tile(const T *data, int64_t num_vecs, int V
     }
 }
 
+// The same copy staged through registers: wave `wib` of WPT loads chunks wib,
+// wib + WPT, ... of tile tl (the placement of copy_tile) into r[] with plain
+// vector loads, and store_tile_regs writes them to LDS later.  Loads into
+// registers can be in flight while the waves read the tile before it from
+// LDS; the compiler orders a read of LDS after a copy_tile into it by waiting
+// for all vector memory, which would serialise the two.
+typedef uint32_t tile_chunk_t __attribute__((ext_vector_type(4))); // 16 bytes, kept in VGPRs
+
+template <typename T, int WPT, int NR>
+__device__ __forceinline__ void load_tile_regs(const T *data, int64_t num_vecs, int V, int C, int64_t tl, int wib,
+                                               int lane, bool swz, tile_chunk_t (&r)[NR])
+{
+    const tile_chunk_t *src = reinterpret_cast<const tile_chunk_t *>(data + tl * kWave * (int64_t)V);
+    const int64_t rows_left = num_vecs - tl * kWave;
+    const int64_t last = (rows_left < kWave ? rows_left : kWave) * C - 1;
+    const float rc = 1.0f / (float)C;
+    // every r[i] is loaded (slots past the row's last chunk load that chunk
+    // again and are never stored), so r[] stays in registers
+#pragma unroll
+    for (int i = 0; i < NR; ++i) {
+        const int k = wib + i * WPT;
+        int g = (k < C ? k : C - 1) * kWave + lane;
+        if (swz) g ^= (int)(((float)g + 0.5f) * rc) & 7;
+        r[i] = src[g < last ? g : last];
+    }
+}
+
+template <int WPT, int NR>
+__device__ __forceinline__ void store_tile_regs(unsigned char *buf, int C, int wib, int lane,
+                                                const tile_chunk_t (&r)[NR])
+{
+#pragma unroll
+    for (int i = 0; i < NR; ++i) {
+        const int k = wib + i * WPT;
+        if (k < C) reinterpret_cast<tile_chunk_t *>(buf)[k * kWave + lane] = r[i];
+    }
+}
+
 // Element e of a 16-byte chunk of 4- or 8-byte words.
 template <typename T>
 __device__ __forceinline__ T chunk_elem(const uint4 &v, int e)
