@@ -432,6 +432,40 @@ def test_empty_tables(ibp, oracle, w):
         assert np.array_equal(_u(_np(km)), okm) and np.array_equal(_u(_np(kb)), okb) and kc.numel() == 0
 
 
+@pytest.mark.parametrize("where,v", [("device", 50), ("pinned", 50), ("device", 130)])
+def test_fetch_captured_in_a_torch_graph(ibp, oracle, where, v):
+    """The plain decompress_fetch with an output tensor allocates nothing and
+    never synchronises, so a torch CUDA graph can capture it: each replay,
+    with the index refilled and eager work and synchronisations between
+    replays, equals the eager fetch and the original rows."""
+    n, w = 3000, 8
+    data, comp, mask, bitval, bitmask, _ = _compressed(ibp, oracle, n, v, w, 77 + v, where)
+    dev = torch.device("cuda")
+    rng = np.random.default_rng(v)
+    idx = torch.from_numpy(rng.integers(0, n, size=1024)).cuda()
+    out = torch.empty((1024, v), dtype=torch.int64, device="cuda")
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):  # first calls (kernel loading) outside the capture
+        ibp.decompress_fetch(comp, mask, bitval, bitmask, dev, None, idx, None, None, None, out)
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        ibp.decompress_fetch(comp, mask, bitval, bitmask, dev, None, idx, None, None, None, out)
+    for rep in range(4):
+        print("graph replay", where, v, rep, flush=True)
+        sel = rng.integers(0, n, size=1024)
+        idx.copy_(torch.from_numpy(sel))
+        out.fill_(-1)
+        g.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(_np(out), data[sel]), rep
+        eager = ibp.decompress_fetch(comp, mask, bitval, bitmask, dev, None, idx)
+        torch.cuda.synchronize()
+        assert torch.equal(eager, out), rep
+
+
 def test_fetch_full_width_fields(ibp, oracle):
     """Raw 64-bit elements at 64-aligned stream positions and flagged elements
     with mask 0 (the full-width shift cases) through compress + fetch."""
