@@ -244,6 +244,16 @@ int ibp_plan_capture_graph(ibp_plan *plan, void *stream);
 int ibp_plan_graph_launch(ibp_plan *plan, void *stream);
 /* Describes the chosen configuration (rows per work item, grid, ...). */
 int ibp_plan_describe(ibp_plan *plan, char *buf, size_t len);
+/* The launch configuration a plan uses (after ibp_plan_autotune: the best
+ * one found) as 7 integers: rows per work item of tables with rows of <= 64
+ * elements and of wider ones, rows in flight per wave (2, 4 or 8), blocks,
+ * work items claimed per queue atomic, dynamic queue (0/1), segmented launch
+ * where the tasks allow it (0/1).  ibp_plan_set_config applies such a
+ * configuration (for instance one an earlier autotune stored) without
+ * timing anything; a value out of range is IBP_ERR_INVALID.  A captured
+ * graph is dropped by set_config and must be captured again. */
+int ibp_plan_get_config(ibp_plan *plan, int32_t cfg[7]);
+int ibp_plan_set_config(ibp_plan *plan, const int32_t cfg[7]);
 
 /* ---- serving (no reference counterpart: BASELINE's served form) -------------- */
 

@@ -70,6 +70,8 @@ SIGNATURES = {
     "ibp_plan_capture_graph": (_I32, [_P, _P]),
     "ibp_plan_graph_launch": (_I32, [_P, _P]),
     "ibp_plan_describe": (_I32, [_P, ctypes.c_char_p, _SZ]),
+    "ibp_plan_get_config": (_I32, [_P, ctypes.POINTER(ctypes.c_int32)]),
+    "ibp_plan_set_config": (_I32, [_P, ctypes.POINTER(ctypes.c_int32)]),
     "ibp_server_create": (_I32, [ctypes.POINTER(ServerConfig), ctypes.POINTER(_P)]),
     "ibp_server_destroy": (_I32, [_P]),
     "ibp_server_register": (_I32, [_P, _I32, _P, _I64, _I64, _P, _P, _P, _I32, _I32, ctypes.POINTER(_I32)]),

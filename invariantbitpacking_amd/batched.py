@@ -6,6 +6,9 @@ captures pointers, so index and output buffers are refilled in place between
 launches; `capture_graph()` records the launch into a HIP graph.
 """
 import ctypes
+import hashlib
+import json
+import os
 
 import torch
 
@@ -57,6 +60,8 @@ class BatchedFetch:
                 1 if comp.is_cuda else 2, None if offsets is None else offsets.data_ptr(),
                 0 if offsets is None else comp.numel() * comp.element_size())
         self._tasks = arr
+        self._signature = [[int(t.elem_bytes), int(t.vec_size), int(t.n), int(t.num_vecs), int(t.index_bytes),
+                            int(t.comp_loc), int(t.comp_len), 1 if t.offsets else 0] for t in arr]
         self._plan = ctypes.c_void_p()
         _capi.call("ibp_plan_create", arr, len(tasks), ctypes.byref(self._plan))
         self.tasks = tasks
@@ -68,10 +73,55 @@ class BatchedFetch:
     def launch(self):
         _capi.call("ibp_plan_launch", self._plan, self._stream())
 
-    def autotune(self):
+    def autotune(self, cache=None):
+        """Times the candidate launch configurations and keeps the fastest;
+        returns its milliseconds per launch.  cache: path of a JSON file of
+        earlier results, keyed by the GPU and the plan's task shapes
+        (element and index widths, row length, rows, where the rows live,
+        comp_len); a hit applies the stored configuration without timing
+        anything, a miss tunes and adds the result."""
+        key = None
+        if cache is not None:
+            key = self._cache_key()
+            stored = self._cache_read(cache).get(key)
+            if stored is not None:
+                self.set_config(stored["config"])
+                return float(stored["ms"])
         ms = ctypes.c_float(0)
         _capi.call("ibp_plan_autotune", self._plan, self._stream(), ctypes.byref(ms))
+        if cache is not None:
+            entries = self._cache_read(cache)
+            entries[key] = {"config": self.config(), "ms": round(float(ms.value), 5), "tasks": len(self._signature),
+                            "device": torch.cuda.get_device_name()}
+            tmp = "%s.tmp%d" % (cache, os.getpid())
+            with open(tmp, "w") as f:
+                json.dump(entries, f, indent=1, sort_keys=True)
+            os.replace(tmp, cache)
         return ms.value
+
+    def config(self):
+        """The plan's launch configuration (see ibp_plan_get_config)."""
+        cfg = (ctypes.c_int32 * 7)()
+        _capi.call("ibp_plan_get_config", self._plan, cfg)
+        return [int(x) for x in cfg]
+
+    def set_config(self, cfg):
+        """Applies a configuration from config() or an autotune cache."""
+        arr = (ctypes.c_int32 * 7)(*[int(x) for x in cfg])
+        _capi.call("ibp_plan_set_config", self._plan, arr)
+
+    def _cache_key(self):
+        blob = json.dumps([torch.cuda.get_device_name(), self._signature], separators=(",", ":"))
+        return hashlib.sha1(blob.encode()).hexdigest()
+
+    @staticmethod
+    def _cache_read(path):
+        try:
+            with open(path) as f:
+                data = json.load(f)
+            return data if isinstance(data, dict) else {}
+        except (OSError, ValueError):
+            return {}
 
     def capture_graph(self):
         _capi.call("ibp_plan_capture_graph", self._plan, self._stream())

@@ -467,9 +467,17 @@ extern "C" int ibp_plan_autotune(ibp_plan *pl, void *stream, float *best_ms)
                 for (int claim : {1, 8})
                     cands.push_back(BatchConfig{rpi, rpi >= 128 ? 4 : 2, rif, dev.num_cus * mult, claim, 1, 0});
             }
-    hipEvent_t e0, e1;
-    IBP_HIP(hipEventCreate(&e0));
-    IBP_HIP(hipEventCreate(&e1));
+    struct Events { // destroyed on every return, the error ones included
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        ~Events()
+        {
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+        }
+    } ev;
+    IBP_HIP(hipEventCreate(&ev.e0));
+    IBP_HIP(hipEventCreate(&ev.e1));
+    hipEvent_t e0 = ev.e0, e1 = ev.e1;
     float best = 1e30f;
     BatchConfig best_cfg = pl->cfg;
     for (const BatchConfig &c : cands) {
@@ -492,8 +500,6 @@ extern "C" int ibp_plan_autotune(ibp_plan *pl, void *stream, float *best_ms)
             best_cfg = c;
         }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     pl->cfg = best_cfg;
     const int st = rebuild(pl);
     if (st != IBP_OK) return st;
@@ -531,6 +537,30 @@ extern "C" int ibp_plan_graph_launch(ibp_plan *pl, void *stream)
     }
     IBP_HIP(hipGraphLaunch(pl->graph_exec, as_stream(stream)));
     return IBP_OK;
+}
+
+extern "C" int ibp_plan_get_config(ibp_plan *pl, int32_t cfg[7])
+{
+    IBP_CHECK(pl != nullptr && cfg != nullptr, "null argument");
+    const BatchConfig &c = pl->cfg;
+    const int32_t v[7] = {c.rows_per_item_small, c.rows_per_item_large, c.rows_in_flight, c.blocks,
+                          c.claim, c.dynamic, c.segmented};
+    for (int k = 0; k < 7; ++k) cfg[k] = v[k];
+    return IBP_OK;
+}
+
+extern "C" int ibp_plan_set_config(ibp_plan *pl, const int32_t cfg[7])
+{
+    IBP_CHECK(pl != nullptr && cfg != nullptr, "null argument");
+    const DeviceInfo &dev = device_info();
+    IBP_CHECK(cfg[0] >= 1 && cfg[0] <= (1 << 20) && cfg[1] >= 1 && cfg[1] <= (1 << 20),
+              "rows per work item must be in [1, 2^20]");
+    IBP_CHECK(cfg[2] == 2 || cfg[2] == 4 || cfg[2] == 8, "rows in flight must be 2, 4 or 8");
+    IBP_CHECK(cfg[3] >= 1 && cfg[3] <= dev.num_cus * 64, "blocks must be in [1, %d]", dev.num_cus * 64);
+    IBP_CHECK(cfg[4] >= 1 && cfg[4] <= 1024, "claim must be in [1, 1024]");
+    IBP_CHECK((cfg[5] == 0 || cfg[5] == 1) && (cfg[6] == 0 || cfg[6] == 1), "dynamic and segmented are 0 or 1");
+    pl->cfg = BatchConfig{cfg[0], cfg[1], cfg[2], cfg[3], cfg[4], cfg[5], cfg[6]};
+    return rebuild(pl);
 }
 
 extern "C" int ibp_plan_describe(ibp_plan *pl, char *buf, size_t len)

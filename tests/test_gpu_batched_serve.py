@@ -64,6 +64,51 @@ def _check(sets, tasks):
         assert torch.equal(t["out"].cpu(), want)
 
 
+def test_autotune_cache_and_config(ibp, sets, tmp_path):
+    """autotune(cache=...) stores the best configuration under a key of the
+    GPU and the task shapes; a plan of the same shapes takes it from the file
+    without timing, and launches it bit-exactly; set_config rejects values out
+    of range."""
+    import json
+    cache = str(tmp_path / "tune.json")
+    tasks = _tasks(sets, 21)
+    bf = ibp.BatchedFetch(tasks)
+    ms = bf.autotune(cache=cache)
+    with open(cache) as f:
+        entries = json.load(f)
+    assert len(entries) == 1
+    (entry,) = entries.values()
+    assert entry["config"] == bf.config() and entry["ms"] == round(ms, 5) and entry["tasks"] == len(sets)
+    bf.launch()
+    torch.cuda.synchronize()
+    _check(sets, tasks)
+    # same shapes, new buffers: a cache hit applies the stored configuration
+    tasks2 = _tasks(sets, 21)
+    bf2 = ibp.BatchedFetch(tasks2)
+    default = bf2.config()
+    assert bf2.autotune(cache=cache) == entry["ms"] and bf2.config() == entry["config"]
+    bf2.launch()
+    torch.cuda.synchronize()
+    _check(sets, tasks2)
+    # other shapes: a second entry
+    bf3 = ibp.BatchedFetch(_tasks(sets[:2], 22))
+    bf3.autotune(cache=cache)
+    with open(cache) as f:
+        assert len(json.load(f)) == 2
+    # configurations by hand; a captured graph is dropped and captured again
+    bf2.set_config(default)
+    assert bf2.config() == default
+    bf2.capture_graph()
+    bf2.graph_launch()
+    torch.cuda.synchronize()
+    _check(sets, tasks2)
+    for bad in ([0] + default[1:], default[:2] + [3] + default[3:], default[:5] + [2, default[6]]):
+        with pytest.raises(RuntimeError):
+            bf2.set_config(bad)
+    for b in (bf, bf2, bf3):
+        b.close()
+
+
 def test_batched_matches_plain(ibp, sets):
     tasks = _tasks(sets, 1)
     bf = ibp.BatchedFetch(tasks)
