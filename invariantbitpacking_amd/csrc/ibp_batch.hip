@@ -520,8 +520,11 @@ extern "C" int ibp_plan_capture_graph(ibp_plan *pl, void *stream)
     const int st = launch(pl, pl->capture_stream);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(pl->capture_stream, &g);
-    if (st != IBP_OK) return st;
-    if (e != hipSuccess) return set_error(IBP_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+    if (st != IBP_OK || e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        if (st != IBP_OK) return st;
+        return set_error(IBP_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+    }
     pl->graph = g;
     IBP_HIP(hipGraphInstantiate(&pl->graph_exec, pl->graph, nullptr, nullptr, 0));
     return IBP_OK;
