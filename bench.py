@@ -170,7 +170,7 @@ def run_batched(ibp, comp, mask, bitval, bitmask, comp_len, idx, x, batch, v):
 
 
 def run_pcie(ibp, comp, mask, bitval, bitmask, comp_len, idx, x, outs, sel, traffic=None):
-    """The GNN minibatch as BASELINE states it: the compressed table in pinned
+    """The GNN minibatch with the compressed table in pinned
     host memory, the kernel reading it over PCIe; every minibatch of the index
     list fetched once (timed), each output checked.  Roofline: compressed
     bytes per launch over kernel time against a measured pinned H2D copy;
@@ -1160,8 +1160,8 @@ def main():
                     help="default run: only the GNN minibatch line, without the other workloads' legs")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(rows shard per GPU over each GPU's own PCIe link; see the design notes)", file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path (rows would shard by range, "
+              "each GPU fetching its share of an index batch over its own PCIe link)", file=sys.stderr)
         sys.exit(2)
     # device_count() does not initialise the GPU: the counter passes of the
     # default run start child processes before this process touches it.

@@ -197,7 +197,7 @@ int ibp_decompress_fetch_ex(int elem_bytes, void *out, const void *comp, int64_t
  * Process-wide; values are unaffected. */
 int ibp_set_fetch_tuning(int rows_in_flight, int blocks_per_cu);
 
-/* ---- batched fetch (no reference counterpart: BASELINE's batched form) ------ */
+/* ---- batched fetch (no reference counterpart: many tables in one launch) ---- */
 
 /* One table fetch of a batch: fields as in ibp_decompress_fetch. */
 typedef struct ibp_fetch_task {
@@ -255,7 +255,7 @@ int ibp_plan_describe(ibp_plan *plan, char *buf, size_t len);
 int ibp_plan_get_config(ibp_plan *plan, int32_t cfg[7]);
 int ibp_plan_set_config(ibp_plan *plan, const int32_t cfg[7]);
 
-/* ---- serving (no reference counterpart: BASELINE's served form) -------------- */
+/* ---- serving (no reference counterpart: continuous batching of requests) ---- */
 
 typedef struct ibp_server ibp_server;
 
