@@ -887,7 +887,8 @@ def run_kernels(args):
     preprocess (sweep), get_compress_size, compress_inplace, the plain fetch of
     200,000 random rows from HBM, the raw-row gather (impl 2) of the same rows
     from the UNCOMPRESSED table, torch's x[idx] gather, compress_condensed of
-    the whole table and the fetch of the same rows from its stream, a 256 MiB
+    the whole table and the fetch of the same rows from its stream, preprocess
+    and k-means(16) of pinned host data (staged in HBM once), a 256 MiB
     device copy
     (the counter calibration for HBM), the same fetch and raw gather from
     pinned memory, and a sequential read of a 256 MiB pinned table (every row
@@ -975,6 +976,36 @@ def run_kernels(args):
                                                         out), dec)
         ok = ok and bool(torch.equal(out, x[idx]))
         del comp_h
+    if not args.no_pcie and (want("preprocess_pinned") or want("kmeans_pinned") or want("preprocess_pinned_direct")):
+        x_h = x.cpu().pin_memory()
+        pm = {}
+
+        def pre_h():
+            pm["mb"] = ibp.preprocess(x_h, None)
+        # the pinned table: copied to HBM once, then both passes read HBM
+        ev("preprocess_pinned", pre_h, x.numel() * 8)
+        if want("preprocess_pinned_direct"):
+            # the C ABI on the pinned table itself: both passes read it over PCIe
+            from invariantbitpacking_amd import _capi
+            md = torch.zeros(v, dtype=torch.int64, device="cuda")
+            bd = torch.zeros(v, dtype=torch.int64, device="cuda")
+            ws_b = _capi.lib().ibp_preprocess_workspace_size(8, v, -1.0)
+            ws = torch.empty(ws_b, dtype=torch.uint8, device="cuda")
+            ev("preprocess_pinned_direct",
+               lambda: _capi.call("ibp_preprocess", 8, x_h.data_ptr(), n, v, -1.0, md.data_ptr(), bd.data_ptr(), None,
+                                  ws.data_ptr(), ws_b, torch.cuda.current_stream().cuda_stream), x.numel() * 8)
+            ok = ok and bool(torch.equal(md, mask)) and bool(torch.equal(bd, bitval))
+        if "mb" in pm:
+            ok = ok and bool(torch.equal(pm["mb"][0], mask)) and bool(torch.equal(pm["mb"][1], bitval))
+        if want("kmeans_pinned"):
+            blk_h = x_h[:4096 * 8].view(4096, -1)  # 4096 rows of 400 elements (an LLM-block-like shape)
+            blk_d = blk_h.cuda()
+            ev("kmeans_pinned", lambda: pm.__setitem__("km", ibp.preprocess_kmeans(blk_h, 16, 0.85, seed=1)),
+               blk_h.numel() * 8)
+            want_km = ibp.preprocess_kmeans(blk_d, 16, 0.85, seed=1)
+            if "km" in pm:
+                ok = ok and all(bool(torch.equal(a.cpu(), b.cpu())) for a, b in zip(pm["km"], want_km))
+        del x_h
     if not args.no_pcie and want("raw_gather_pinned"):
         x_h = x.cpu().pin_memory()
         ev("raw_gather_pinned", lambda: ibp.decompress_fetch(x_h, mask, bitval, zero, dev, v, idx, None, None, 2,

@@ -253,6 +253,25 @@ Tensor fetch_output(at::ScalarType dtype, int64_t n, int64_t v, const c10::Devic
     return pinned_empty({n, v}, dtype);
 }
 
+// Passes that read a table several times (preprocess: the count and the
+// score; k-means: two per iteration, twenty in all) read pinned host data
+// over PCIe each time.  When the table fits in half the free device memory
+// it is copied to the GPU once, on the current stream, and the passes read
+// that copy from HBM; the values do not change.  The copy is released on
+// return, after the stream-ordered kernels that read it were queued.
+Tensor staged_for_passes(const Tensor &dataset)
+{
+    if (dataset.is_cuda() || dataset.numel() == 0) return dataset;
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+        (void)hipGetLastError();
+        return dataset;
+    }
+    const size_t bytes = (size_t)dataset.numel() * (size_t)dataset.element_size();
+    if (bytes > free_b / 2) return dataset;
+    return dataset.to(at::TensorOptions().device(at::kCUDA), /*non_blocking=*/true);
+}
+
 // ---- bindings -----------------------------------------------------------------
 
 // print_debug_msg (src/ibp_api.cu:9-12).
@@ -274,7 +293,8 @@ std::tuple<Tensor, Tensor> preprocess(const Tensor &dataset, std::optional<doubl
     Tensor bitval = at::zeros({v}, on_gpu(signed_type(w)));
     const size_t ws_bytes = ibp_preprocess_workspace_size((int)w, v, t);
     Tensor ws = at::empty({(int64_t)ws_bytes}, on_gpu(at::kByte));
-    check_status(ibp_preprocess((int)w, ptr(dataset), n, v, t, ptr(mask), ptr(bitval), nullptr, ptr(ws), ws_bytes,
+    const Tensor data = staged_for_passes(dataset);
+    check_status(ibp_preprocess((int)w, ptr(data), n, v, t, ptr(mask), ptr(bitval), nullptr, ptr(ws), ws_bytes,
                                 cur_stream()));
     return {mask, bitval};
 }
@@ -297,11 +317,12 @@ std::tuple<Tensor, Tensor, Tensor> preprocess_kmeans(const Tensor &dataset, int6
     Tensor bitvals = at::zeros({num_clusters, v}, on_gpu(signed_type(w)));
     Tensor clusters = pinned_empty({n}, at::kInt).zero_();
     const float t = threshold.has_value() ? (float)*threshold : -1.0f;
+    const Tensor data = staged_for_passes(dataset);
     void *st = cur_stream();
     int rc;
     {
         py::gil_scoped_release nogil;  // the call synchronises its stream
-        rc = ibp_preprocess_kmeans((int)w, ptr(dataset), n, v, (int)num_clusters, t, seed.has_value() ? *seed : -1,
+        rc = ibp_preprocess_kmeans((int)w, ptr(data), n, v, (int)num_clusters, t, seed.has_value() ? *seed : -1,
                                    ptr(masks), ptr(bitvals), static_cast<int32_t *>(clusters.data_ptr()), st);
     }
     check_status(rc);
