@@ -31,14 +31,20 @@ namespace ibp {
 template <typename T>
 using KW = typename std::conditional<sizeof(T) == 8, uint64_t, uint32_t>::type;
 
+// Centroids of the lane-per-row assignment (K <= kRowsK) are also kept by
+// column: cent_cols[j * kRowsK + c] = centroid c's word j (zero for c >= K).
+constexpr int kRowsK = 16;
+
 template <typename T>
-__global__ void km_gather_kernel(const T *data, int64_t V, const int64_t *rows, int K, T *centroids)
+__global__ void km_gather_kernel(const T *data, int64_t V, const int64_t *rows, int K, T *centroids, T *cent_cols)
 {
     const int64_t total = (int64_t)K * V;
     for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (int64_t)gridDim.x * blockDim.x) {
         const int64_t c = g / V, j = g % V;
         const int64_t r = rows[c];
-        centroids[g] = r >= 0 ? data[r * V + j] : T(0);
+        const T v = r >= 0 ? data[r * V + j] : T(0);
+        centroids[g] = v;
+        if (cent_cols) cent_cols[j * kRowsK + c] = v;
     }
 }
 
@@ -58,6 +64,22 @@ constexpr int kAssignGroup = 16;
 constexpr int kAssignRows = 4;
 constexpr int kAssignWaves = 4;
 static_assert(kAssignGroup * kAssignRows == 64, "one distance per lane after the wave sum");
+
+// d + popc(v), one v_bcnt_u32_b32 per 32-bit half with d as its addend.
+__device__ __forceinline__ int popc_add(uint64_t v, int d)
+{
+    d += __builtin_popcount((uint32_t)v);
+    return d + __builtin_popcount((uint32_t)(v >> 32));
+}
+__device__ __forceinline__ int popc_add(uint32_t v, int d) { return d + __builtin_popcount(v); }
+
+// The W at byte `off` of `base`.
+template <typename W>
+__device__ __forceinline__ W load_at(const void *base, uint32_t off)
+{
+    return *reinterpret_cast<const W *>(static_cast<const char *>(base) + off);
+}
+
 template <typename T, int SPLIT>
 __global__ __launch_bounds__(kWave * kAssignWaves) void km_assign_kernel(const T *data, int64_t N, int64_t V,
                                                                          const T *centroids, int K, int32_t *cluster)
@@ -80,16 +102,42 @@ __global__ __launch_bounds__(kWave * kAssignWaves) void km_assign_kernel(const T
             int d[kAssignGroup * kAssignRows];
 #pragma unroll
             for (int q = 0; q < kAssignGroup * kAssignRows; ++q) d[q] = 0;
-            for (int64_t j = (int64_t)sl * kWave + lane; j < V; j += kWave * SPLIT) {
-                KW<T> x[kAssignRows];
+            // Addresses are a row's or a centroid's base (the same for the
+            // whole wave, so scalar) plus the lane's 32-bit byte offset: the
+            // host keeps V * sizeof(T) below 2^24.
+            const T *cb = centroids + (int64_t)c0 * V;
+            const uint32_t row_bytes = (uint32_t)(V * (int64_t)sizeof(T));
+            const uint32_t off0 = (uint32_t)((sl * kWave + lane) * (int)sizeof(T));
+            constexpr uint32_t kStep = kWave * SPLIT * sizeof(T);
+            if (c0 + kAssignGroup <= K) {
+                // a whole group: the loads of a column step are issued before
+                // its arithmetic
+                for (uint32_t off = off0; off < row_bytes; off += kStep) {
+                    KW<T> x[kAssignRows], cw[kAssignGroup];
 #pragma unroll
-                for (int r = 0; r < kAssignRows; ++r) x[r] = (KW<T>)data[row[r] * V + j];
+                    for (int r = 0; r < kAssignRows; ++r) x[r] = load_at<KW<T>>(data + row[r] * V, off);
 #pragma unroll
-                for (int g = 0; g < kAssignGroup; ++g) {
-                    if (c0 + g >= K) break; // uniform
-                    const KW<T> cw = (KW<T>)centroids[(int64_t)(c0 + g) * V + j];
+                    for (int g = 0; g < kAssignGroup; ++g) cw[g] = load_at<KW<T>>(cb + (int64_t)g * V, off);
 #pragma unroll
-                    for (int r = 0; r < kAssignRows; ++r) d[g * kAssignRows + r] += popc((KW<T>)(x[r] ^ cw));
+                    for (int g = 0; g < kAssignGroup; ++g)
+#pragma unroll
+                        for (int r = 0; r < kAssignRows; ++r)
+                            d[g * kAssignRows + r] = popc_add((KW<T>)(x[r] ^ cw[g]), d[g * kAssignRows + r]);
+                }
+            } else {
+                for (uint32_t off = off0; off < row_bytes; off += kStep) {
+                    KW<T> x[kAssignRows];
+#pragma unroll
+                    for (int r = 0; r < kAssignRows; ++r) x[r] = load_at<KW<T>>(data + row[r] * V, off);
+#pragma unroll
+                    for (int g = 0; g < kAssignGroup; ++g) {
+                        if (c0 + g < K) { // uniform
+                            const KW<T> cw = load_at<KW<T>>(cb + (int64_t)g * V, off);
+#pragma unroll
+                            for (int r = 0; r < kAssignRows; ++r)
+                                d[g * kAssignRows + r] = popc_add((KW<T>)(x[r] ^ cw), d[g * kAssignRows + r]);
+                        }
+                    }
                 }
             }
             int dist = wave_sum64(d);
@@ -119,11 +167,204 @@ __global__ __launch_bounds__(kWave * kAssignWaves) void km_assign_kernel(const T
     }
 }
 
+// Assignment with a lane per row, for K <= kRowsK centroids (kRowsK
+// distances are computed; those past K are never chosen).
+// Block (t, g) of a (row tiles) x (column groups) grid takes rows
+// 64 t .. 64 t + 63 and the column group's words.  The block copies
+// 64 x kRowsTileBytes of its rows at a time to LDS (each load instruction
+// reads four 128-byte row pieces; the next tile is loaded into registers
+// while the current one is used), then each of its four waves takes a quarter
+// of the tile's words: lane r reads row r's word from LDS and the sixteen
+// centroid words of that column, consecutive in cent_cols, are the same for
+// the whole wave, so they are scalar loads used as instruction operands.  A
+// word costs an XOR and a popcount-accumulate per 32 bits and centroid; no
+// lane exchange is needed.  The four waves' distances are summed in LDS.
+// With one column group the block picks each row's nearest centroid;
+// otherwise it stores its partial distances to part[g][row][c] for
+// km_argmin_kernel.
+constexpr int kRowsWaves = 4;
+constexpr int kRowsTileBytes = 128; // of each row per tile
+
+// d + popc(v) as one v_bcnt_u32_b32 with d as its addend.
+__device__ __forceinline__ int bcnt_acc(uint32_t v, int d)
+{
+    asm("v_bcnt_u32_b32 %0, %1, %0" : "+v"(d) : "v"(v));
+    return d;
+}
+__device__ __forceinline__ int hamming_acc(uint64_t x, uint64_t c, int d)
+{
+    return bcnt_acc((uint32_t)(x >> 32) ^ (uint32_t)(c >> 32), bcnt_acc((uint32_t)x ^ (uint32_t)c, d));
+}
+__device__ __forceinline__ int hamming_acc(uint32_t x, uint32_t c, int d) { return bcnt_acc(x ^ c, d); }
+
+// The kRowsK centroid words of one column (consecutive in cent_cols), read
+// as 64-byte vectors so that a uniform address becomes s_load_dwordx16.
+typedef uint32_t dword16_t __attribute__((ext_vector_type(16)));
+template <typename W>
+struct CentColumn {
+    static constexpr int kVecs = kRowsK * (int)sizeof(W) / 64;
+    dword16_t v[kVecs];
+    __device__ __forceinline__ explicit CentColumn(const W *col)
+    {
+#pragma unroll
+        for (int i = 0; i < kVecs; ++i) v[i] = reinterpret_cast<const dword16_t *>(col)[i];
+    }
+    __device__ __forceinline__ W operator[](int c) const
+    {
+        if constexpr (sizeof(W) == 8) {
+            const dword16_t &h = v[c / 8];
+            return ((uint64_t)h[2 * (c % 8) + 1] << 32) | h[2 * (c % 8)];
+        } else {
+            return v[0][c];
+        }
+    }
+};
+
+// Nearest of the first K distances; equal distances go to the highest index.
+__device__ __forceinline__ int32_t nearest(const int (&d)[kRowsK], int K)
+{
+    int best = 0, best_d = 0x7fffffff;
+#pragma unroll
+    for (int c = 0; c < kRowsK; ++c)
+        if (c < K && d[c] <= best_d) {
+            best_d = d[c];
+            best = c;
+        }
+    return best;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kWave * kRowsWaves) void km_assign_rows_kernel(const T *__restrict__ data, int64_t N,
+                                                                            int64_t V, const T *__restrict__ cent_cols,
+                                                                            int K, int64_t group_words,
+                                                                            int32_t *__restrict__ cluster,
+                                                                            int32_t *__restrict__ part)
+{
+    using W = KW<T>;
+    constexpr int TW = kRowsTileBytes / (int)sizeof(W); // words of a row per tile
+    constexpr int WPW = TW / kRowsWaves;                // of them per wave
+    constexpr int LOADS = kWave * TW / (kWave * kRowsWaves);
+    __shared__ W tile[kWave][TW + 1];
+    __shared__ int sum[kRowsWaves - 1][kRowsK][kWave];
+    const W *__restrict__ cent = reinterpret_cast<const W *>(cent_cols);
+    const int lane = lane_id();
+    const int wv = wave_in_block_id();
+    const int64_t row0 = (int64_t)blockIdx.x * kWave;
+    const int64_t col0 = (int64_t)blockIdx.y * group_words;
+    const int64_t col1 = col0 + group_words < V ? col0 + group_words : V;
+    // the words this thread copies: tile row q / TW, word q % TW, q = tid + i * block
+    const W *src[LOADS];
+    int dst_row[LOADS], dst_word[LOADS];
+#pragma unroll
+    for (int i = 0; i < LOADS; ++i) {
+        const int q = (int)threadIdx.x + i * kWave * kRowsWaves;
+        dst_row[i] = q / TW;
+        dst_word[i] = q % TW;
+        int64_t r = row0 + dst_row[i];
+        if (r >= N) r = N - 1; // rows past N copy the last row; their result is dropped
+        src[i] = reinterpret_cast<const W *>(data) + r * V + dst_word[i];
+    }
+    int d[kRowsK];
+#pragma unroll
+    for (int c = 0; c < kRowsK; ++c) d[c] = 0;
+    W next[LOADS];
+    {
+        const int valid = (int)(col1 - col0 < TW ? col1 - col0 : TW);
+#pragma unroll
+        for (int i = 0; i < LOADS; ++i) next[i] = dst_word[i] < valid ? src[i][col0] : W(0);
+    }
+    for (int64_t j0 = col0; j0 < col1; j0 += TW) {
+        const int valid = (int)(col1 - j0 < TW ? col1 - j0 : TW);
+        __syncthreads(); // the previous tile has been read
+#pragma unroll
+        for (int i = 0; i < LOADS; ++i) tile[dst_row[i]][dst_word[i]] = next[i];
+        __syncthreads();
+        if (j0 + TW < col1) {
+            const int64_t j1 = j0 + TW;
+            const int valid1 = (int)(col1 - j1 < TW ? col1 - j1 : TW);
+#pragma unroll
+            for (int i = 0; i < LOADS; ++i) next[i] = dst_word[i] < valid1 ? src[i][j1] : W(0);
+        }
+        const int w0 = wv * WPW;
+        const int w1 = w0 + WPW < valid ? w0 + WPW : valid;
+        int w = w0;
+        for (; w + 2 <= w1; w += 2) { // two columns' loads in flight together
+            const W x0 = tile[lane][w], x1 = tile[lane][w + 1];
+            const CentColumn<W> c0(cent + (j0 + w) * kRowsK), c1(cent + (j0 + w + 1) * kRowsK);
+#pragma unroll
+            for (int c = 0; c < kRowsK; ++c) d[c] = hamming_acc(x0, c0[c], d[c]);
+#pragma unroll
+            for (int c = 0; c < kRowsK; ++c) d[c] = hamming_acc(x1, c1[c], d[c]);
+        }
+        if (w < w1) {
+            const W x = tile[lane][w];
+            const CentColumn<W> cw(cent + (j0 + w) * kRowsK);
+#pragma unroll
+            for (int c = 0; c < kRowsK; ++c) d[c] = hamming_acc(x, cw[c], d[c]);
+        }
+    }
+    // waves 1..3 hand their distances to wave 0
+    if (wv > 0) {
+#pragma unroll
+        for (int c = 0; c < kRowsK; ++c) sum[wv - 1][c][lane] = d[c];
+    }
+    __syncthreads();
+    if (wv != 0) return;
+#pragma unroll
+    for (int w = 0; w < kRowsWaves - 1; ++w)
+#pragma unroll
+        for (int c = 0; c < kRowsK; ++c) d[c] += sum[w][c][lane];
+    const int64_t r = row0 + lane;
+    if (r >= N) return;
+    if (gridDim.y == 1) {
+        cluster[r] = nearest(d, K);
+    } else {
+        int4 *p = reinterpret_cast<int4 *>(part + ((int64_t)blockIdx.y * gridDim.x * kWave + r) * kRowsK);
+#pragma unroll
+        for (int c = 0; c < kRowsK; c += 4) p[c / 4] = make_int4(d[c], d[c + 1], d[c + 2], d[c + 3]);
+    }
+}
+
+// Sums the column groups' partial distances of each row and picks its
+// nearest centroid (km_assign_rows_kernel with more than one group): thread
+// (row, c) sums distance c over the groups (the 16 threads of a row read 64
+// consecutive bytes per group), then the row's 16 lanes take the minimum of
+// (distance, 15 - c), so equal distances go to the highest index.
+__global__ __launch_bounds__(256) void km_argmin_kernel(int64_t N, int K, int groups, int64_t group_stride,
+                                                        const int32_t *__restrict__ part, int32_t *__restrict__ cluster)
+{
+    const int c = (int)(threadIdx.x % kRowsK);
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < N * kRowsK;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = t / kRowsK;
+        const int32_t *p = part + r * kRowsK + c;
+        const int64_t gs = group_stride * kRowsK;
+        int a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+        int g = 0;
+        for (; g + 4 <= groups; g += 4) {
+            a0 += p[(int64_t)g * gs];
+            a1 += p[(int64_t)(g + 1) * gs];
+            a2 += p[(int64_t)(g + 2) * gs];
+            a3 += p[(int64_t)(g + 3) * gs];
+        }
+        for (; g < groups; ++g) a0 += p[(int64_t)g * gs];
+        const int dist = (a0 + a1) + (a2 + a3);
+        int key = c < K ? (dist << 4) | (kRowsK - 1 - c) : 0x7fffffff;
+#pragma unroll
+        for (int m = 1; m < kRowsK; m <<= 1) {
+            const int o = (int)shfl((uint32_t)key, lane_id() ^ m);
+            key = o < key ? o : key;
+        }
+        if (c == 0) cluster[r] = kRowsK - 1 - (key & (kRowsK - 1));
+    }
+}
+
 // Rows of each cluster, grouped (a counting sort of the assignment) in one
 // block of 1024 threads with the histogram in LDS: count[c], offset[c] and
 // perm[offset[c] ..) = the rows of cluster c (in no particular order).  For
 // K <= kSortMaxK; larger K use the three-kernel form below.
 constexpr int kSortMaxK = 4096;
+constexpr int kSortUnroll = 4;
 //
 // It also cuts every cluster into chunks of at most `chunk_rows` rows for the
 // bit count: chunk e = (cluster, first row, end row) in chunks[3e..3e+2],
@@ -136,9 +377,33 @@ __global__ __launch_bounds__(1024) void km_sort_kernel(const int32_t *cluster, i
 {
     __shared__ int32_t hist[kSortMaxK];
     __shared__ int32_t base[kSortMaxK];
+    // Rows are read kSortUnroll at a time (their loads in flight together);
+    // the first kSortUnroll * blockDim.x rows' clusters stay in registers for
+    // the scatter.
+    constexpr int U = kSortUnroll;
+    const int64_t step = (int64_t)U * blockDim.x;
+    int first[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int64_t i = threadIdx.x + (int64_t)u * blockDim.x;
+        first[u] = i < N ? cluster[i] : -1;
+    }
     for (int c = threadIdx.x; c < K; c += blockDim.x) hist[c] = 0;
     __syncthreads();
-    for (int64_t i = threadIdx.x; i < N; i += blockDim.x) atomicAdd(&hist[cluster[i]], 1);
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+        if (first[u] >= 0) atomicAdd(&hist[first[u]], 1);
+    for (int64_t i0 = step + threadIdx.x; i0 < N; i0 += step) {
+        int cu[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t i = i0 + (int64_t)u * blockDim.x;
+            cu[u] = i < N ? cluster[i] : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (cu[u] >= 0) atomicAdd(&hist[cu[u]], 1);
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         int32_t acc = 0, e = 0;
@@ -163,9 +428,21 @@ __global__ __launch_bounds__(1024) void km_sort_kernel(const int32_t *cluster, i
         hist[c] = 0; // becomes the fill counter
     }
     __syncthreads();
-    for (int64_t i = threadIdx.x; i < N; i += blockDim.x) {
-        const int c = cluster[i];
-        perm[base[c] + atomicAdd(&hist[c], 1)] = (int32_t)i;
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+        if (first[u] >= 0)
+            perm[base[first[u]] + atomicAdd(&hist[first[u]], 1)] = (int32_t)(threadIdx.x + u * blockDim.x);
+    for (int64_t i0 = step + threadIdx.x; i0 < N; i0 += step) {
+        int cu[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t i = i0 + (int64_t)u * blockDim.x;
+            cu[u] = i < N ? cluster[i] : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (cu[u] >= 0)
+                perm[base[cu[u]] + atomicAdd(&hist[cu[u]], 1)] = (int32_t)(i0 + (int64_t)u * blockDim.x);
     }
 }
 
@@ -318,6 +595,43 @@ __global__ void km_reduce_bits_kernel(int64_t VB, int K, const int32_t *chunk_ba
     }
 }
 
+// km_reduce_bits_kernel and km_majority_kernel in one pass: a wave sums the
+// chunk counts of 64 consecutive (column, bit) counters of one cluster (one
+// 8-byte column or two 4-byte ones), lane l the counter l, and the majority
+// words are one ballot of (count > members / 2).  The summed counts are
+// stored only when `store_bits` (the last iteration, whose counts give the
+// masks); the other iterations need only the majority words.
+template <typename T>
+__global__ __launch_bounds__(256) void km_reduce_majority_kernel(int64_t V, int K, const int32_t *count,
+                                                                 const int32_t *chunk_base,
+                                                                 const int32_t *chunk_count, const int32_t *part,
+                                                                 int store_bits, int32_t *bits_set, T *maj)
+{
+    constexpr int B = 8 * (int)sizeof(T);
+    const int64_t VB = V * B;
+    const int64_t waves_per_cluster = VB / kWave; // V * B is a multiple of 64 (B = 32 needs even V: see launch)
+    const int64_t total = (int64_t)K * waves_per_cluster;
+    const int lane = lane_id();
+    const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x / kWave);
+    for (int64_t gw = global_wave_id(); gw < total; gw += nwaves) {
+        const int c = (int)(gw / waves_per_cluster);
+        const int64_t i = (gw % waves_per_cluster) * kWave + lane; // counter (column j = i / B, bit i % B)
+        const int32_t e0 = chunk_base[c], ne = chunk_count[c];
+        const int32_t *p = part + (int64_t)e0 * VB + i;
+        int32_t acc = 0;
+        int32_t e = 0;
+        for (; e + 4 <= ne; e += 4) {
+            const int32_t a0 = p[0], a1 = p[VB], a2 = p[2 * VB], a3 = p[3 * VB];
+            acc += (a0 + a1) + (a2 + a3);
+            p += 4 * VB;
+        }
+        for (; e < ne; ++e, p += VB) acc += p[0];
+        if (store_bits) bits_set[(int64_t)c * VB + i] = acc;
+        const uint64_t m = ballot(acc > count[c] / 2);
+        if (lane % B == 0) maj[((int64_t)c * VB + i) / B] = (T)(m >> lane);
+    }
+}
+
 // Majority word of cluster c, column j: bit k set iff more than half of the
 // cluster's members have it (the per-column part of construct_bitmasks).
 template <typename T>
@@ -335,34 +649,43 @@ __global__ void km_majority_kernel(int64_t V, int K, const int32_t *count, const
     maj[g] = (T)acc;
 }
 
-// construct_bitmasks (ibp_preproc_kmeans.cuh:219-236): thread (c, l), l < 32,
-// walks columns l, l+32, ... and ORs majority bits into one accumulator that
-// is never cleared, so centroid[c][l + 32i] = OR of maj[c][l + 32i'], i' <= i.
-// The majority words come from km_majority_kernel; the chain's loads are
-// issued 8 at a time.
+// construct_bitmasks (ibp_preproc_kmeans.cuh:219-236): chain l < 32 of
+// centroid c walks columns l, l+32, ... and ORs majority bits into one
+// accumulator that is never cleared, so
+// centroid[c][l + 32i] = OR of maj[c][l + 32i'], i' <= i (a prefix OR).
+// A block per centroid: thread (p, l) takes part p of 32 consecutive parts of
+// chain l, ORs its words, the parts' ORs meet in LDS, and the thread writes
+// its part's prefix ORs starting from the OR of the parts before it.  The
+// majority words come from km_majority_kernel or km_reduce_majority_kernel;
+// with cent_cols the centroid is also stored by column.
 template <typename T>
-__global__ void km_construct_kernel(int64_t V, int K, const T *maj, T *centroids)
+__global__ __launch_bounds__(1024) void km_construct_kernel(int64_t V, int K, const T *maj, T *centroids,
+                                                            T *cent_cols)
 {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= K * 32) return;
-    const int c = g / 32, l = g % 32;
+    __shared__ T part_or[32][32];
+    const int c = blockIdx.x;
+    const int l = (int)threadIdx.x % 32, p = (int)threadIdx.x / 32;
+    const int64_t chain = (V + 31) / 32;
+    const int64_t per = (chain + 31) / 32;
+    const int64_t i0 = p * per, i1 = i0 + per < chain ? i0 + per : chain;
     const T *src = maj + (int64_t)c * V;
     T *dst = centroids + (int64_t)c * V;
     T acc = 0;
-    int64_t id = l;
-    for (; id + 32 * 7 < V; id += 32 * 8) {
-        T w[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) w[u] = src[id + 32 * u];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            acc = (T)(acc | w[u]);
-            dst[id + 32 * u] = acc;
-        }
+    for (int64_t i = i0; i < i1; ++i) {
+        const int64_t j = l + 32 * i;
+        if (j < V) acc = (T)(acc | src[j]);
     }
-    for (; id < V; id += 32) {
-        acc = (T)(acc | src[id]);
-        dst[id] = acc;
+    part_or[p][l] = acc;
+    __syncthreads();
+    T run = 0;
+    for (int q = 0; q < p; ++q) run = (T)(run | part_or[q][l]);
+    for (int64_t i = i0; i < i1; ++i) {
+        const int64_t j = l + 32 * i;
+        if (j < V) {
+            run = (T)(run | src[j]);
+            dst[j] = run;
+            if (cent_cols) cent_cols[j * kRowsK + c] = run;
+        }
     }
 }
 
@@ -444,7 +767,8 @@ int kmeans_typed(const void *data_v, int64_t N, int64_t V, int K, int64_t seed, 
         libstdcxx_random_shuffle(idx);
         for (int64_t c = 1; c < (K < N ? K : N); ++c) init[(size_t)c] = idx[(size_t)c];
     }
-    T *cent = nullptr, *maj = nullptr;
+    T *cent = nullptr, *maj = nullptr, *cent_cols = nullptr;
+    int32_t *dist = nullptr;
     int32_t *cluster = nullptr, *count = nullptr, *offset = nullptr, *fill = nullptr, *perm = nullptr, *bits = nullptr;
     int32_t *chunks = nullptr, *part = nullptr;
     int64_t *rows = nullptr;
@@ -461,6 +785,8 @@ int kmeans_typed(const void *data_v, int64_t N, int64_t V, int K, int64_t seed, 
     const int64_t max_chunks = chunked ? N / chunk_rows + K + 1 : 0;
     auto cleanup = [&]() {
         (void)hipFreeAsync(cent, s);
+        (void)hipFreeAsync(cent_cols, s);
+        (void)hipFreeAsync(dist, s);
         (void)hipFreeAsync(maj, s);
         (void)hipFreeAsync(cluster, s);
         (void)hipFreeAsync(count, s);
@@ -471,7 +797,25 @@ int kmeans_typed(const void *data_v, int64_t N, int64_t V, int K, int64_t seed, 
         (void)hipFreeAsync(part, s);
     };
     hipError_t e = hipSuccess;
+    // Lane-per-row assignment (K <= kRowsK): row tiles x column groups, about
+    // eight blocks per CU (2 to 8 measured alike on 4096-row blocks), and the
+    // centroids also by column.
+    const bool rows_assign = K <= kRowsK;
+    constexpr int64_t kTileWords = kRowsTileBytes / (int64_t)sizeof(KW<T>);
+    const int64_t row_tiles = (N + kWave - 1) / kWave;
+    int64_t col_groups = 1, group_words = V;
+    if (rows_assign && row_tiles > 0) {
+        const int64_t max_groups = (V + kTileWords - 1) / kTileWords;
+        col_groups = ((int64_t)dev.num_cus * 8 + row_tiles - 1) / row_tiles;
+        if (col_groups > max_groups) col_groups = max_groups;
+        if (col_groups < 1) col_groups = 1;
+        group_words = ((V + col_groups - 1) / col_groups + kTileWords - 1) / kTileWords * kTileWords;
+        col_groups = (V + group_words - 1) / group_words;
+    }
+    const size_t dist_bytes = col_groups > 1 ? sizeof(int32_t) * (size_t)(col_groups * row_tiles * kWave * kRowsK) : 0;
     if ((e = hipMallocAsync((void **)&cent, sizeof(T) * (size_t)(K * V), s)) != hipSuccess ||
+        (rows_assign && (e = hipMallocAsync((void **)&cent_cols, sizeof(T) * (size_t)(kRowsK * V), s)) != hipSuccess) ||
+        (dist_bytes && (e = hipMallocAsync((void **)&dist, dist_bytes, s)) != hipSuccess) ||
         (e = hipMallocAsync((void **)&maj, sizeof(T) * (size_t)(K * V), s)) != hipSuccess ||
         (e = hipMallocAsync((void **)&cluster, sizeof(int32_t) * (size_t)(N > 0 ? N : 1), s)) != hipSuccess ||
         (e = hipMallocAsync((void **)&count, sizeof(int32_t) * 3 * (size_t)K, s)) != hipSuccess ||
@@ -487,7 +831,9 @@ int kmeans_typed(const void *data_v, int64_t N, int64_t V, int K, int64_t seed, 
     offset = count + K;
     fill = count + 2 * K;
     IBP_HIP(hipMemcpyAsync(rows, init.data(), sizeof(int64_t) * (size_t)K, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL((km_gather_kernel<T>), dim3(blocks_for((int64_t)K * V)), dim3(256), 0, s, data, V, rows, K, cent);
+    if (rows_assign) IBP_HIP(hipMemsetAsync(cent_cols, 0, sizeof(T) * (size_t)(kRowsK * V), s));
+    hipLaunchKernelGGL((km_gather_kernel<T>), dim3(blocks_for((int64_t)K * V)), dim3(256), 0, s, data, V, rows, K, cent,
+                       cent_cols);
     hipLaunchKernelGGL(km_zero_kernel, dim3(blocks_for(N)), dim3(256), 0, s, cluster, N);
     IBP_LAUNCH_CHECK();
 
@@ -508,8 +854,19 @@ int kmeans_typed(const void *data_v, int64_t N, int64_t V, int K, int64_t seed, 
     int32_t *chunk_base = chunked ? chunks + 3 * max_chunks : nullptr;
     int32_t *chunk_count = chunked ? chunk_base + K : nullptr;
     int32_t *nchunks = chunked ? chunk_count + K : nullptr;
+    // chunk sums and majority words in one kernel when every wave's 64
+    // counters lie in one cluster
+    const bool fused_majority = chunked && (V * B) % kWave == 0;
     for (int it = 0; it < 10; ++it) {
-        if (assign_split)
+        if (rows_assign) {
+            if (N > 0)
+                hipLaunchKernelGGL((km_assign_rows_kernel<T>), dim3((unsigned)row_tiles, (unsigned)col_groups),
+                                   dim3(kWave * kRowsWaves), 0, s, data, N, V, (const T *)cent_cols, K, group_words,
+                                   cluster, dist);
+            if (col_groups > 1)
+                hipLaunchKernelGGL(km_argmin_kernel, dim3(blocks_for(N * kRowsK)), dim3(256), 0, s, N, K, (int)col_groups,
+                                   row_tiles * kWave, (const int32_t *)dist, cluster);
+        } else if (assign_split)
             hipLaunchKernelGGL((km_assign_kernel<T, kAssignWaves>), dim3((unsigned)assign_grid),
                                dim3(kWave * kAssignWaves), 0, s, data, N, V, cent, K, cluster);
         else
@@ -520,8 +877,15 @@ int kmeans_typed(const void *data_v, int64_t N, int64_t V, int K, int64_t seed, 
                                (int)chunk_rows, chunks, chunk_base, chunk_count, nchunks);
             hipLaunchKernelGGL((km_bitcount_chunks_kernel<T>), dim3(tiles, (unsigned)max_chunks), dim3(256), 0, s,
                                data, V, perm, (const int32_t *)chunks, (const int32_t *)nchunks, part);
-            hipLaunchKernelGGL(km_reduce_bits_kernel, dim3(blocks_for(nbits, 1 << 16)), dim3(256), 0, s, V * B, K,
-                               (const int32_t *)chunk_base, (const int32_t *)chunk_count, (const int32_t *)part, bits);
+            if (fused_majority) {
+                hipLaunchKernelGGL((km_reduce_majority_kernel<T>), dim3(blocks_for(nbits, 1 << 16)), dim3(256), 0,
+                                   s, V, K, (const int32_t *)count, (const int32_t *)chunk_base,
+                                   (const int32_t *)chunk_count, (const int32_t *)part, it == 9 ? 1 : 0, bits, maj);
+            } else {
+                hipLaunchKernelGGL(km_reduce_bits_kernel, dim3(blocks_for(nbits, 1 << 16)), dim3(256), 0, s, V * B,
+                                   K, (const int32_t *)chunk_base, (const int32_t *)chunk_count,
+                                   (const int32_t *)part, bits);
+            }
         } else {
             hipLaunchKernelGGL(km_zero_kernel, dim3(1), dim3(256), 0, s, count, (int64_t)K);
             hipLaunchKernelGGL(km_hist_kernel, dim3(blocks_for(N)), dim3(256), 0, s, cluster, N, count);
@@ -532,10 +896,11 @@ int kmeans_typed(const void *data_v, int64_t N, int64_t V, int K, int64_t seed, 
             hipLaunchKernelGGL((km_bitcount_kernel<T>), dim3(tiles, (unsigned)K, splits), dim3(256), 0, s, data, V,
                                perm, offset, count, bits);
         }
-        hipLaunchKernelGGL((km_majority_kernel<T>), dim3(blocks_for((int64_t)K * V, 1 << 20)), dim3(256), 0, s, V, K,
-                           count, bits, maj);
-        hipLaunchKernelGGL((km_construct_kernel<T>), dim3((unsigned)((K * 32 + 255) / 256)), dim3(256), 0, s, V, K,
-                           (const T *)maj, cent);
+        if (!fused_majority)
+            hipLaunchKernelGGL((km_majority_kernel<T>), dim3(blocks_for((int64_t)K * V, 1 << 20)), dim3(256), 0, s, V,
+                               K, count, bits, maj);
+        hipLaunchKernelGGL((km_construct_kernel<T>), dim3((unsigned)K), dim3(1024), 0, s, V, K, (const T *)maj, cent,
+                           cent_cols);
         e = hipGetLastError();
         if (e != hipSuccess) {
             cleanup();
@@ -570,6 +935,9 @@ extern "C" int ibp_preprocess_kmeans(int elem_bytes, const void *data, int64_t n
     IBP_CHECK(vec_size > 0 && num_vecs >= 0, "invalid shape");
     IBP_CHECK(num_clusters >= 1 && num_clusters <= 65535, "num_clusters must be in [1, 65535]");
     IBP_CHECK(num_vecs < (int64_t)1 << 31, "num_vecs must fit int32 cluster ids and counters");
+    // distances (at most 8 * vec_size * elem_bytes bits) carry a 4-bit
+    // centroid tag in an int32
+    IBP_CHECK(vec_size * elem_bytes < (int64_t)1 << 24, "rows of 16 MiB or more are not supported");
     IBP_CHECK(masks && bitvals && ((data && clusters) || num_vecs == 0), "null pointer argument");
     const int st = elem_bytes == 8
                        ? kmeans_typed<uint64_t>(data, num_vecs, vec_size, num_clusters, seed, masks, bitvals, clusters,
