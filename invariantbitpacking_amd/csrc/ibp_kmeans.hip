@@ -557,14 +557,25 @@ __global__ __launch_bounds__(256) void km_bitcount_chunks_kernel(const T *data, 
 #pragma unroll
     for (int k = 0; k < B; ++k) acc[k] = 0;
     VCounter<W, B> vc;
-    for (int64_t q = q0 + wv; q < q1; q += (int64_t)CHUNK * waves) {
-        W x[CHUNK];
+    // The wave's rows are q0 + wv, q0 + wv + waves, ...: their ids come 64 at a
+    // time in one vector load (lane l holds row l), and 32 rows' words are
+    // loaded together before they are counted.
+    for (int64_t qw = q0 + wv; qw < q1; qw += (int64_t)kWave * waves) {
+        const int64_t ql = qw + (int64_t)lane * waves;
+        const int32_t rid = ql < q1 ? perm[ql] : 0;
+        const int64_t left = (q1 - qw + waves - 1) / waves;
+        const int nr = left < kWave ? (int)left : kWave;
+        for (int b = 0; b < nr; b += 2 * CHUNK) {
+            W xa[CHUNK], xb[CHUNK];
 #pragma unroll
-        for (int u = 0; u < CHUNK; ++u) {
-            const int64_t qi = q + (int64_t)u * waves;
-            x[u] = (col && qi < q1) ? (W)data[(int64_t)perm[qi] * V + j] : W(0);
+            for (int u = 0; u < CHUNK; ++u) {
+                const int64_t ra = readlane(rid, b + u), rb = readlane(rid, b + CHUNK + u);
+                xa[u] = (col && b + u < nr) ? (W)data[ra * V + j] : W(0);
+                xb[u] = (col && b + CHUNK + u < nr) ? (W)data[rb * V + j] : W(0);
+            }
+            vc.add16(xa, acc);
+            vc.add16(xb, acc);
         }
-        vc.add16(x, acc);
     }
     vc.flush(acc);
     __syncthreads();
