@@ -46,6 +46,13 @@ __global__ void km_gather_kernel(const T *data, int64_t V, const int64_t *rows, 
         centroids[g] = v;
         if (cent_cols) cent_cols[j * kRowsK + c] = v;
     }
+    // the by-column copy's centroids past K are zero
+    if (cent_cols && K < kRowsK) {
+        const int pad = kRowsK - K;
+        for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < pad * V;
+             g += (int64_t)gridDim.x * blockDim.x)
+            cent_cols[(g / pad) * kRowsK + K + g % pad] = T(0);
+    }
 }
 
 // Nearest centroid by popc(x ^ c); equal distances go to the highest index.
@@ -794,19 +801,6 @@ int kmeans_typed(const void *data_v, int64_t N, int64_t V, int K, int64_t seed, 
     int64_t chunk_rows = 256;
     if (chunked && N / chunk_rows + K + 1 > max_chunks_mem) chunk_rows = (N + max_chunks_mem - K - 1) / (max_chunks_mem - K);
     const int64_t max_chunks = chunked ? N / chunk_rows + K + 1 : 0;
-    auto cleanup = [&]() {
-        (void)hipFreeAsync(cent, s);
-        (void)hipFreeAsync(cent_cols, s);
-        (void)hipFreeAsync(dist, s);
-        (void)hipFreeAsync(maj, s);
-        (void)hipFreeAsync(cluster, s);
-        (void)hipFreeAsync(count, s);
-        (void)hipFreeAsync(perm, s);
-        (void)hipFreeAsync(bits, s);
-        (void)hipFreeAsync(rows, s);
-        (void)hipFreeAsync(chunks, s);
-        (void)hipFreeAsync(part, s);
-    };
     hipError_t e = hipSuccess;
     // Lane-per-row assignment (K <= kRowsK): row tiles x column groups, about
     // eight blocks per CU (2 to 8 measured alike on 4096-row blocks), and the
@@ -823,30 +817,57 @@ int kmeans_typed(const void *data_v, int64_t N, int64_t V, int K, int64_t seed, 
         group_words = ((V + col_groups - 1) / col_groups + kTileWords - 1) / kTileWords * kTileWords;
         col_groups = (V + group_words - 1) / group_words;
     }
-    const size_t dist_bytes = col_groups > 1 ? sizeof(int32_t) * (size_t)(col_groups * row_tiles * kWave * kRowsK) : 0;
-    if ((e = hipMallocAsync((void **)&cent, sizeof(T) * (size_t)(K * V), s)) != hipSuccess ||
-        (rows_assign && (e = hipMallocAsync((void **)&cent_cols, sizeof(T) * (size_t)(kRowsK * V), s)) != hipSuccess) ||
-        (dist_bytes && (e = hipMallocAsync((void **)&dist, dist_bytes, s)) != hipSuccess) ||
-        (e = hipMallocAsync((void **)&maj, sizeof(T) * (size_t)(K * V), s)) != hipSuccess ||
-        (e = hipMallocAsync((void **)&cluster, sizeof(int32_t) * (size_t)(N > 0 ? N : 1), s)) != hipSuccess ||
-        (e = hipMallocAsync((void **)&count, sizeof(int32_t) * 3 * (size_t)K, s)) != hipSuccess ||
-        (e = hipMallocAsync((void **)&perm, sizeof(int32_t) * (size_t)(N > 0 ? N : 1), s)) != hipSuccess ||
-        (e = hipMallocAsync((void **)&bits, sizeof(int32_t) * (size_t)nbits, s)) != hipSuccess ||
-        (e = hipMallocAsync((void **)&rows, sizeof(int64_t) * (size_t)K, s)) != hipSuccess ||
-        (chunked && (e = hipMallocAsync((void **)&chunks, sizeof(int32_t) * (size_t)(3 * max_chunks + 2 * K + 1), s)) !=
-                        hipSuccess) ||
-        (chunked && (e = hipMallocAsync((void **)&part, (size_t)(per_chunk * max_chunks), s)) != hipSuccess)) {
-        cleanup();
+    // One workspace allocation, cut into 256-byte aligned buffers.
+    const int64_t n1 = N > 0 ? N : 1;
+    const size_t sz_cent = sizeof(T) * (size_t)(K * V);
+    const size_t sz_cols = rows_assign ? sizeof(T) * (size_t)(kRowsK * V) : 0;
+    const size_t sz_dist = col_groups > 1 ? sizeof(int32_t) * (size_t)(col_groups * row_tiles * kWave * kRowsK) : 0;
+    const size_t sz_cluster = sizeof(int32_t) * (size_t)n1;
+    const size_t sz_count = sizeof(int32_t) * 3 * (size_t)K;
+    const size_t sz_bits = sizeof(int32_t) * (size_t)nbits;
+    const size_t sz_rows = sizeof(int64_t) * (size_t)K;
+    const size_t sz_chunks = chunked ? sizeof(int32_t) * (size_t)(3 * max_chunks + 2 * K + 1) : 0;
+    const size_t sz_part = chunked ? (size_t)(per_chunk * max_chunks) : 0;
+    const size_t sizes[] = {sz_cent, sz_cols, sz_dist, sz_cent, sz_cluster, sz_count, sz_cluster, sz_bits, sz_rows,
+                            sz_chunks, sz_part};
+    size_t total = 0;
+    for (size_t b : sizes) total += (b + 255) / 256 * 256;
+    unsigned char *ws = nullptr;
+    if ((e = hipMallocAsync((void **)&ws, total, s)) != hipSuccess)
         return set_error(IBP_ERR_NOMEM, "kmeans workspace allocation failed: %s", hipGetErrorString(e));
+    {
+        size_t at = 0;
+        auto take = [&](size_t b) -> void * {
+            void *p = b ? ws + at : nullptr;
+            at += (b + 255) / 256 * 256;
+            return p;
+        };
+        cent = (T *)take(sz_cent);
+        cent_cols = (T *)take(sz_cols);
+        dist = (int32_t *)take(sz_dist);
+        maj = (T *)take(sz_cent);
+        cluster = (int32_t *)take(sz_cluster);
+        count = (int32_t *)take(sz_count);
+        perm = (int32_t *)take(sz_cluster);
+        bits = (int32_t *)take(sz_bits);
+        rows = (int64_t *)take(sz_rows);
+        chunks = (int32_t *)take(sz_chunks);
+        part = (int32_t *)take(sz_part);
     }
+    auto cleanup = [&]() { (void)hipFreeAsync(ws, s); };
     offset = count + K;
     fill = count + 2 * K;
-    IBP_HIP(hipMemcpyAsync(rows, init.data(), sizeof(int64_t) * (size_t)K, hipMemcpyHostToDevice, s));
-    if (rows_assign) IBP_HIP(hipMemsetAsync(cent_cols, 0, sizeof(T) * (size_t)(kRowsK * V), s));
+    if ((e = hipMemcpyAsync(rows, init.data(), sizeof(int64_t) * (size_t)K, hipMemcpyHostToDevice, s)) != hipSuccess) {
+        cleanup();
+        return set_error(IBP_ERR_HIP, "kmeans initialisation failed: %s", hipGetErrorString(e));
+    }
     hipLaunchKernelGGL((km_gather_kernel<T>), dim3(blocks_for((int64_t)K * V)), dim3(256), 0, s, data, V, rows, K, cent,
                        cent_cols);
     hipLaunchKernelGGL(km_zero_kernel, dim3(blocks_for(N)), dim3(256), 0, s, cluster, N);
-    IBP_LAUNCH_CHECK();
+    if ((e = hipGetLastError()) != hipSuccess) {
+        cleanup();
+        return set_error(IBP_ERR_HIP, "kmeans initialisation failed: %s", hipGetErrorString(e));
+    }
 
     const bool assign_split = V >= kWave * kAssignWaves;
     const int64_t groups = (N + kAssignRows - 1) / kAssignRows;
