@@ -377,6 +377,58 @@ constexpr int kSortUnroll = 4;
 // bit count: chunk e = (cluster, first row, end row) in chunks[3e..3e+2],
 // cluster c's chunks are chunk_base[c] .. chunk_base[c] + chunk_count[c] - 1,
 // and nchunks[0] is the total.
+// Counting-sort steps for the lanes of one wave holding cluster ids c (-1:
+// none).  With few clusters (K <= 64) many lanes may share an id, and
+// atomics on one LDS word are serialised: up to kSortPeel times the wave
+// takes its lowest pending lane's id, ballots the lanes with it, and one lane
+// adds their number; lanes still pending then add their own.  (Peeling every
+// id costs more than it saves when a wave holds many different ids, and the
+// sort runs on one CU.)
+constexpr int kSortPeel = 4;
+__device__ __forceinline__ void wave_count(int32_t *hist, int c, bool aggregate)
+{
+    bool mine = c >= 0;
+    if (aggregate) {
+        uint64_t todo = ballot(mine);
+        for (int it = 0; it < kSortPeel && todo; ++it) {
+            const int leader = __builtin_ctzll(todo);
+            const int v = readlane(c, leader);
+            const uint64_t same = ballot(c == v) & todo;
+            if (lane_id() == leader) atomicAdd(&hist[v], (int)__builtin_popcountll(same));
+            if (c == v) mine = false;
+            todo &= ~same;
+        }
+    }
+    if (mine) atomicAdd(&hist[c], 1);
+}
+
+// base[c] + a slot among cluster c's rows (fill[c] counts the slots taken).
+__device__ __forceinline__ int wave_claim(const int32_t *base, int32_t *fill, int c, bool aggregate)
+{
+    int slot = -1;
+    bool mine = c >= 0;
+    if (aggregate) {
+        const int lane = lane_id();
+        const uint64_t below = (1ull << lane) - 1;
+        uint64_t todo = ballot(mine);
+        for (int it = 0; it < kSortPeel && todo; ++it) {
+            const int leader = __builtin_ctzll(todo);
+            const int v = readlane(c, leader);
+            const uint64_t same = ballot(c == v) & todo;
+            int first = 0;
+            if (lane == leader) first = base[v] + atomicAdd(&fill[v], (int)__builtin_popcountll(same));
+            first = readlane(first, leader);
+            if (c == v) {
+                slot = first + (int)__builtin_popcountll(same & below);
+                mine = false;
+            }
+            todo &= ~same;
+        }
+    }
+    if (mine) slot = base[c] + atomicAdd(&fill[c], 1);
+    return slot;
+}
+
 __global__ __launch_bounds__(1024) void km_sort_kernel(const int32_t *cluster, int64_t N, int K, int32_t *count,
                                                        int32_t *offset, int32_t *perm, int chunk_rows,
                                                        int32_t *chunks, int32_t *chunk_base, int32_t *chunk_count,
@@ -395,12 +447,15 @@ __global__ __launch_bounds__(1024) void km_sort_kernel(const int32_t *cluster, i
         const int64_t i = threadIdx.x + (int64_t)u * blockDim.x;
         first[u] = i < N ? cluster[i] : -1;
     }
+    const bool aggregate = K <= kWave;
     for (int c = threadIdx.x; c < K; c += blockDim.x) hist[c] = 0;
     __syncthreads();
 #pragma unroll
-    for (int u = 0; u < U; ++u)
-        if (first[u] >= 0) atomicAdd(&hist[first[u]], 1);
-    for (int64_t i0 = step + threadIdx.x; i0 < N; i0 += step) {
+    for (int u = 0; u < U; ++u) wave_count(hist, first[u], aggregate);
+    // the block's threads run the same number of iterations (whole waves
+    // take part in the ballots)
+    for (int64_t b0 = step; b0 < N; b0 += step) {
+        const int64_t i0 = b0 + threadIdx.x;
         int cu[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -408,8 +463,7 @@ __global__ __launch_bounds__(1024) void km_sort_kernel(const int32_t *cluster, i
             cu[u] = i < N ? cluster[i] : -1;
         }
 #pragma unroll
-        for (int u = 0; u < U; ++u)
-            if (cu[u] >= 0) atomicAdd(&hist[cu[u]], 1);
+        for (int u = 0; u < U; ++u) wave_count(hist, cu[u], aggregate);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -436,10 +490,12 @@ __global__ __launch_bounds__(1024) void km_sort_kernel(const int32_t *cluster, i
     }
     __syncthreads();
 #pragma unroll
-    for (int u = 0; u < U; ++u)
-        if (first[u] >= 0)
-            perm[base[first[u]] + atomicAdd(&hist[first[u]], 1)] = (int32_t)(threadIdx.x + u * blockDim.x);
-    for (int64_t i0 = step + threadIdx.x; i0 < N; i0 += step) {
+    for (int u = 0; u < U; ++u) {
+        const int slot = wave_claim(base, hist, first[u], aggregate);
+        if (slot >= 0) perm[slot] = (int32_t)(threadIdx.x + u * blockDim.x);
+    }
+    for (int64_t b0 = step; b0 < N; b0 += step) {
+        const int64_t i0 = b0 + threadIdx.x;
         int cu[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -447,9 +503,10 @@ __global__ __launch_bounds__(1024) void km_sort_kernel(const int32_t *cluster, i
             cu[u] = i < N ? cluster[i] : -1;
         }
 #pragma unroll
-        for (int u = 0; u < U; ++u)
-            if (cu[u] >= 0)
-                perm[base[cu[u]] + atomicAdd(&hist[cu[u]], 1)] = (int32_t)(i0 + (int64_t)u * blockDim.x);
+        for (int u = 0; u < U; ++u) {
+            const int slot = wave_claim(base, hist, cu[u], aggregate);
+            if (slot >= 0) perm[slot] = (int32_t)(i0 + (int64_t)u * blockDim.x);
+        }
     }
 }
 
