@@ -381,11 +381,7 @@ __device__ __forceinline__ void fetch_small_range_r(const FetchParams &p, const 
         // (the common case) runs without per-row branches, so the rows' scans
         // and exchanges interleave; any other group takes the general form.
         const uint64_t all = nrows == ROWS ? ((1ull << ROWS) - 1) : 0ull;
-#ifdef IBP_NO_BRANCH_FREE_GROUPS
-        if (false)
-#else
         if (all && (okmask & pkmask & all) == all)
-#endif
             small_unpack_rows<T, ROWS, RUNS, NT, true>(p, col, w, cur, base, ROWS, okmask, pkmask, len_flag, len_raw, Hw, E,
                                                    comp, out);
         else
