@@ -516,6 +516,18 @@ def test_kmeans_matches_oracle(ibp, oracle, w, n, v, k):
         assert np.array_equal(_u(_np(m)), om) and np.array_equal(_u(_np(b)), ob)
 
 
+def test_kmeans_or_chain_known_answer_on_gpu(ibp):
+    """The hand-derived stride-32 OR-chain case (33 columns, K = 2): the
+    never-cleared accumulator carries column 0's majority word into column 32
+    and moves row 1 to cluster 1."""
+    from test_oracle import or_chain_data, or_chain_expected
+    em, eb, ecl = or_chain_expected()
+    for where in ("device", "pinned"):
+        m, b, cl = ibp.preprocess_kmeans(place(or_chain_data(), where), 2, seed=1)
+        assert np.array_equal(cl.numpy(), ecl)
+        assert np.array_equal(_np(m).view(np.uint32), em) and np.array_equal(_np(b).view(np.uint32), eb)
+
+
 def test_kmeans_mask_row0_used_by_compress(ibp, oracle):
     """A [K, V] mask is used through its first row, as in the reference."""
     data = float_rows(64, 16, 8, seed=1)
