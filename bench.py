@@ -1124,7 +1124,7 @@ def run_llm(args):
                             "fetch_pinned_ms_per_layer": round(c_ms / len(layers), 3),
                             "fetch_pinned_decompressed_gbps": round(dec_bytes(raw, c_ms), 2),
                             "fetch_pinned_pcie_gbps": round(dec_bytes(comp_c, c_ms), 2), "bitexact": c_exact},
-            "compression_ratio": round(raw / comp_b, 4), "kmeans_s_per_block": round(t_km / len(blocks), 3),
+            "compression_ratio": round(raw / comp_b, 4), "kmeans_ms_per_block": round(t_km / len(blocks) * 1e3, 3),
             "compress_inplace_pinned_gbps": round(raw / t_pack / 1e9, 2),
             "fetch_pinned_ms_per_layer": round(f_ms / len(layers), 3),
             "fetch_pinned_decompressed_gbps": round(dec / (f_ms / 1e3) / 1e9, 2),
