@@ -761,12 +761,12 @@ __device__ __forceinline__ void pack_chunk(const uint4 &v, const PackChunkCols<T
 {
     constexpr int B = Bits<T>::B;
     constexpr int EPC = PackChunkCols<T, R>::EPC;
+    uint32_t cf = 0; // the chunk's flags, bit e for element e, shifted into fl once
 #pragma unroll
     for (int e = 0; e < EPC; ++e) {
-        const int j = c * EPC + e;
         const T x = chunk_elem<T>(v, e);
         const bool fs = (T)(x & k.m[e]) == k.b[e];
-        st.fl |= fs ? (uint64_t(1) << j) : uint64_t(0);
+        cf |= fs ? (1u << e) : 0u;
         const int L = fs ? (int)k.len[e] : B;
         T ext;
         if constexpr (R < 0) {
@@ -787,6 +787,7 @@ __device__ __forceinline__ void pack_chunk(const uint4 &v, const PackChunkCols<T
         st.wpos += full ? 1 : 0;
         st.nb = t & (B - 1);
     }
+    st.fl |= (uint64_t)cf << (c * EPC);
 }
 
 struct PackTileParams {
