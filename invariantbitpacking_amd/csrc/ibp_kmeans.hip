@@ -510,29 +510,115 @@ __global__ __launch_bounds__(1024) void km_sort_kernel(const int32_t *cluster, i
     }
 }
 
-__global__ void km_hist_kernel(const int32_t *cluster, int64_t N, int32_t *count)
+// Beyond the one-block sort (more than 2^20 rows): histogram and scatter
+// over many blocks.  Each block counts its rows per cluster in LDS (wave-
+// aggregated atomics) and adds one global atomic per cluster it holds, so a
+// few clusters do not serialise millions of global atomics on one word.
+// Blocks of 256 threads; K <= kSortMaxK (larger K take the per-row form).
+__global__ __launch_bounds__(256) void km_hist_kernel(const int32_t *cluster, int64_t N, int K, int32_t *count)
 {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x)
-        atomicAdd(&count[cluster[i]], 1);
+    __shared__ int32_t local[kSortMaxK];
+    const bool in_lds = K <= kSortMaxK;
+    const bool aggregate = K <= kWave;
+    if (in_lds) {
+        for (int c = threadIdx.x; c < K; c += blockDim.x) local[c] = 0;
+        __syncthreads();
+    }
+    // whole waves take part in every step (the ballots of wave_count)
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t b0 = (int64_t)blockIdx.x * blockDim.x; b0 < N; b0 += stride) {
+        const int64_t i = b0 + threadIdx.x;
+        const int c = i < N ? cluster[i] : -1;
+        if (in_lds)
+            wave_count(local, c, aggregate);
+        else if (c >= 0)
+            atomicAdd(&count[c], 1);
+    }
+    if (!in_lds) return;
+    __syncthreads();
+    for (int c = threadIdx.x; c < K; c += blockDim.x)
+        if (local[c]) atomicAdd(&count[c], local[c]);
 }
 
-__global__ void km_offsets_kernel(const int32_t *count, int K, int32_t *offset, int32_t *fill)
+// offset[c] (exclusive scan of the counts), fill[c] = 0, and with
+// chunk_base: cluster c's chunks of at most chunk_rows rows are
+// chunk_base[c] .. chunk_base[c] + chunk_count[c] - 1, nchunks[0] in all.
+__global__ void km_offsets_kernel(const int32_t *count, int K, int32_t *offset, int32_t *fill, int chunk_rows,
+                                  int32_t *chunk_base, int32_t *chunk_count, int32_t *nchunks)
 {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    int32_t acc = 0;
+    int32_t acc = 0, e = 0;
     for (int c = 0; c < K; ++c) {
         offset[c] = acc;
         fill[c] = 0;
         acc += count[c];
+        if (chunk_base) {
+            const int32_t nc = (count[c] + chunk_rows - 1) / chunk_rows;
+            chunk_base[c] = e;
+            chunk_count[c] = nc;
+            e += nc;
+        }
     }
+    if (chunk_base) nchunks[0] = e;
 }
 
-__global__ void km_scatter_kernel(const int32_t *cluster, int64_t N, const int32_t *offset, int32_t *fill,
-                                  int32_t *perm)
+// The chunk list of km_sort_kernel's form (chunks[3e..3e+2] = cluster, first
+// and end position in perm) from km_offsets_kernel's chunk counts: thread e
+// finds its cluster, the last one whose chunks start at or before e.
+__global__ void km_chunks_kernel(const int32_t *count, const int32_t *offset, const int32_t *chunk_base,
+                                 const int32_t *nchunks, int K, int chunk_rows, int64_t max_chunks, int32_t *chunks)
 {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
-        const int c = cluster[i];
-        perm[offset[c] + atomicAdd(&fill[c], 1)] = (int32_t)i;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= max_chunks || e >= nchunks[0]) return;
+    int lo = 0, hi = K - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (chunk_base[mid] <= e)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const int c = lo;
+    const int32_t b = (int32_t)(e - chunk_base[c]) * chunk_rows;
+    const int32_t n = count[c];
+    chunks[3 * e] = c;
+    chunks[3 * e + 1] = offset[c] + b;
+    chunks[3 * e + 2] = offset[c] + (b + chunk_rows < n ? b + chunk_rows : n);
+}
+
+// perm[offset[c] ..) = the rows of cluster c.  A block counts its rows per
+// cluster in LDS, reserves one range per cluster with a global atomic, then
+// places its rows in that range (a second pass over its cluster ids).
+__global__ __launch_bounds__(256) void km_scatter_kernel(const int32_t *cluster, int64_t N, int K,
+                                                         const int32_t *offset, int32_t *fill, int32_t *perm)
+{
+    __shared__ int32_t local[kSortMaxK];
+    __shared__ int32_t base[kSortMaxK];
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    if (K > kSortMaxK) {
+        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += stride) {
+            const int c = cluster[i];
+            perm[offset[c] + atomicAdd(&fill[c], 1)] = (int32_t)i;
+        }
+        return;
+    }
+    const bool aggregate = K <= kWave;
+    for (int c = threadIdx.x; c < K; c += blockDim.x) local[c] = 0;
+    __syncthreads();
+    for (int64_t b0 = (int64_t)blockIdx.x * blockDim.x; b0 < N; b0 += stride) {
+        const int64_t i = b0 + threadIdx.x;
+        wave_count(local, i < N ? cluster[i] : -1, aggregate);
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < K; c += blockDim.x) {
+        base[c] = local[c] ? offset[c] + atomicAdd(&fill[c], local[c]) : 0;
+        local[c] = 0; // becomes the block's fill counter
+    }
+    __syncthreads();
+    for (int64_t b0 = (int64_t)blockIdx.x * blockDim.x; b0 < N; b0 += stride) {
+        const int64_t i = b0 + threadIdx.x;
+        const int slot = wave_claim(base, local, i < N ? cluster[i] : -1, aggregate);
+        if (slot >= 0) perm[slot] = (int32_t)i;
     }
 }
 
@@ -805,14 +891,24 @@ float sweep_last_threshold()
     return last;
 }
 
-// libstdc++ random_shuffle(first, last) over 0..n-1 with rand().
-void libstdcxx_random_shuffle(std::vector<int32_t> &a)
+// Positions 0..m-1 of libstdc++'s random_shuffle(first, last) over 0..n-1
+// with rand(): every rand() call of the whole shuffle is made, but only the
+// first m positions are kept.  Step i swaps positions i and j <= i, and
+// position i still holds i then (no earlier step reaches past its own
+// index), so for i >= m the step only sets position j to i when j < m.
+std::vector<int32_t> random_shuffle_head(int64_t n, int64_t m)
 {
-    const int64_t n = (int64_t)a.size();
+    std::vector<int32_t> a((size_t)m);
+    for (int64_t i = 0; i < m; ++i) a[(size_t)i] = (int32_t)i;
     for (int64_t i = 1; i < n; ++i) {
         const int64_t j = (int64_t)(rand() % (int)(i + 1));
-        if (i != j) std::swap(a[i], a[j]);
+        if (i < m) {
+            if (i != j) std::swap(a[(size_t)i], a[(size_t)j]);
+        } else if (j < m) {
+            a[(size_t)j] = (int32_t)i;
+        }
     }
+    return a;
 }
 
 namespace {
@@ -837,10 +933,9 @@ int kmeans_typed(const void *data_v, int64_t N, int64_t V, int K, int64_t seed, 
     if (N > 0) init[0] = 0;
     if (seed >= 0) srand((unsigned)seed);
     {
-        std::vector<int32_t> idx((size_t)N);
-        for (int64_t i = 0; i < N; ++i) idx[(size_t)i] = (int32_t)i;
-        libstdcxx_random_shuffle(idx);
-        for (int64_t c = 1; c < (K < N ? K : N); ++c) init[(size_t)c] = idx[(size_t)c];
+        const int64_t m = K < N ? K : N;
+        const std::vector<int32_t> head = random_shuffle_head(N, m);
+        for (int64_t c = 1; c < m; ++c) init[(size_t)c] = head[(size_t)c];
     }
     T *cent = nullptr, *maj = nullptr, *cent_cols = nullptr;
     int32_t *dist = nullptr;
@@ -849,13 +944,26 @@ int kmeans_typed(const void *data_v, int64_t N, int64_t V, int K, int64_t seed, 
     int64_t *rows = nullptr;
     const int64_t nbits = (int64_t)K * V * B;
     // The LDS counting sort (one block) serves K <= kSortMaxK and up to 2^20
-    // rows; it also cuts the clusters into chunks of chunk_rows rows whose
-    // partial bit counts (at most 256 MiB of them) are summed per cluster.
+    // rows, a histogram and scatter over many blocks more rows; both cut the
+    // clusters into chunks of chunk_rows rows whose partial bit counts (about
+    // 256 MiB of them at most) are summed per cluster.
     const bool sort_in_lds = K <= kSortMaxK && N <= ((int64_t)1 << 20);
     const int64_t per_chunk = V * B * (int64_t)sizeof(int32_t);
-    const int64_t max_chunks_mem = ((int64_t)256 << 20) / per_chunk;
-    const bool chunked = sort_in_lds && max_chunks_mem > K;
+    // (the chunk index is blockIdx.y of the bit count: below 65535)
+    int64_t max_chunks_mem = ((int64_t)256 << 20) / per_chunk;
+    if (max_chunks_mem > 65534) max_chunks_mem = 65534;
+    const bool chunked = K <= kSortMaxK && max_chunks_mem > K;
+    // 256-row chunks for the one-block sort; beyond it, about eight bit-count
+    // blocks per CU (more, smaller chunks cost more in the per-cluster sums
+    // than they gain)
     int64_t chunk_rows = 256;
+    if (chunked && !sort_in_lds) {
+        const int64_t tiles64 = (V + kWave - 1) / kWave;
+        int64_t want = (int64_t)dev.num_cus * 8 / tiles64;
+        if (want < K + 1) want = K + 1;
+        const int64_t rows = (N + want - 1) / want;
+        if (rows > chunk_rows) chunk_rows = rows;
+    }
     if (chunked && N / chunk_rows + K + 1 > max_chunks_mem) chunk_rows = (N + max_chunks_mem - K - 1) / (max_chunks_mem - K);
     const int64_t max_chunks = chunked ? N / chunk_rows + K + 1 : 0;
     hipError_t e = hipSuccess;
@@ -962,8 +1070,21 @@ int kmeans_typed(const void *data_v, int64_t N, int64_t V, int K, int64_t seed, 
             hipLaunchKernelGGL((km_assign_kernel<T, 1>), dim3((unsigned)assign_grid), dim3(kWave * kAssignWaves), 0,
                                s, data, N, V, cent, K, cluster);
         if (chunked) {
-            hipLaunchKernelGGL(km_sort_kernel, dim3(1), dim3(1024), 0, s, cluster, N, K, count, offset, perm,
-                               (int)chunk_rows, chunks, chunk_base, chunk_count, nchunks);
+            if (sort_in_lds) {
+                hipLaunchKernelGGL(km_sort_kernel, dim3(1), dim3(1024), 0, s, cluster, N, K, count, offset, perm,
+                                   (int)chunk_rows, chunks, chunk_base, chunk_count, nchunks);
+            } else {
+                // more than 2^20 rows: the same grouping and chunk list over
+                // many blocks
+                hipLaunchKernelGGL(km_zero_kernel, dim3(1), dim3(256), 0, s, count, (int64_t)K);
+                hipLaunchKernelGGL(km_hist_kernel, dim3(blocks_for(N)), dim3(256), 0, s, cluster, N, K, count);
+                hipLaunchKernelGGL(km_offsets_kernel, dim3(1), dim3(64), 0, s, count, K, offset, fill,
+                                   (int)chunk_rows, chunk_base, chunk_count, nchunks);
+                hipLaunchKernelGGL(km_chunks_kernel, dim3(blocks_for(max_chunks, 1 << 20)), dim3(256), 0, s, count,
+                                   offset, chunk_base, nchunks, K, (int)chunk_rows, max_chunks, chunks);
+                hipLaunchKernelGGL(km_scatter_kernel, dim3(blocks_for(N)), dim3(256), 0, s, cluster, N, K, offset,
+                                   fill, perm);
+            }
             hipLaunchKernelGGL((km_bitcount_chunks_kernel<T>), dim3(tiles, (unsigned)max_chunks), dim3(256), 0, s,
                                data, V, perm, (const int32_t *)chunks, (const int32_t *)nchunks, part);
             if (fused_majority) {
@@ -977,9 +1098,10 @@ int kmeans_typed(const void *data_v, int64_t N, int64_t V, int K, int64_t seed, 
             }
         } else {
             hipLaunchKernelGGL(km_zero_kernel, dim3(1), dim3(256), 0, s, count, (int64_t)K);
-            hipLaunchKernelGGL(km_hist_kernel, dim3(blocks_for(N)), dim3(256), 0, s, cluster, N, count);
-            hipLaunchKernelGGL(km_offsets_kernel, dim3(1), dim3(64), 0, s, count, K, offset, fill);
-            hipLaunchKernelGGL(km_scatter_kernel, dim3(blocks_for(N)), dim3(256), 0, s, cluster, N, offset, fill,
+            hipLaunchKernelGGL(km_hist_kernel, dim3(blocks_for(N)), dim3(256), 0, s, cluster, N, K, count);
+            hipLaunchKernelGGL(km_offsets_kernel, dim3(1), dim3(64), 0, s, count, K, offset, fill, 1,
+                               (int32_t *)nullptr, (int32_t *)nullptr, (int32_t *)nullptr);
+            hipLaunchKernelGGL(km_scatter_kernel, dim3(blocks_for(N)), dim3(256), 0, s, cluster, N, K, offset, fill,
                                perm);
             if (splits > 1) hipLaunchKernelGGL(km_zero_kernel, dim3(blocks_for(nbits)), dim3(256), 0, s, bits, nbits);
             hipLaunchKernelGGL((km_bitcount_kernel<T>), dim3(tiles, (unsigned)K, splits), dim3(256), 0, s, data, V,

@@ -494,17 +494,20 @@ def test_fetch_full_width_fields(ibp, oracle):
 @pytest.mark.parametrize("w,n,v,k", [(4, 60, 40, 4), (8, 60, 70, 3), (8, 300, 64, 16), (4, 10, 5, 16),
                                      (8, 4096, 130, 16), (4, 3000, 300, 20), (8, 6000, 64, 4),
                                      (4, 1100000, 2, 3), (8, 300000, 20, 12), (8, 2048, 520, 17),
-                                     (4, 1, 1, 1), (8, 1, 3, 2), (8, 65, 17, 16), (4, 129, 33, 5)])
+                                     (4, 1, 1, 1), (8, 1, 3, 2), (8, 65, 17, 16), (4, 129, 33, 5),
+                                     (8, 1200000, 8, 16), (4, 5000, 2, 4100)])
 def test_kmeans_matches_oracle(ibp, oracle, w, n, v, k):
     """Small shapes; 4096 x 130 (one block per cluster and column tile; the
     row-per-lane assignment over nine column groups, the last one two words
     wide, summed by the argmin kernel); K = 20 (the wave-per-row-group
-    assignment, two groups of 16 centroids); 6000 rows in 4 clusters (row
-    splits adding into zeroed counters); 1.1M rows (the counting sort over
-    several kernels instead of one block); 300,000 8-byte rows (one column
-    group: the assignment kernel picks the nearest centroid itself); 8-byte
-    rows with K = 17 (the wave-per-row-group assignment, a second group of
-    one centroid); one row, K > N, partial row tiles and odd widths."""
+    assignment, two groups of 16 centroids); 6000 rows in 4 clusters; 1.1M
+    and 1.2M rows (beyond the one-block sort: histogram, chunk list and
+    scatter over many blocks); 300,000 8-byte rows (one column group: the
+    assignment kernel picks the nearest centroid itself); 8-byte rows with
+    K = 17 (the wave-per-row-group assignment, a second group of one
+    centroid); K = 4100 (beyond the LDS histograms: per-row atomics and a
+    bit count per cluster into zeroed counters); one row, K > N, partial row
+    tiles and odd widths."""
     rng = np.random.default_rng(n + v + k)
     dt = np.dtype("u%d" % w)
     base = rng.integers(0, 1 << 62, size=(4, v), dtype=np.uint64).astype(dt)
