@@ -1040,7 +1040,8 @@ int kmeans_typed(const void *data_v, int64_t N, int64_t V, int K, int64_t seed, 
     if (assign_grid > (int64_t)dev.num_cus * 8) assign_grid = (int64_t)dev.num_cus * 8;
     if (assign_grid < 1) assign_grid = 1;
     const unsigned tiles = (unsigned)((V + kWave - 1) / kWave);
-    // Without the chunk list (very large N or K): row splits of the bit count,
+    // Without the chunk list (K > kSortMaxK, or rows so wide that K chunks of
+    // partial counts exceed the 256 MiB bound): row splits of the bit count,
     // about 64 rows of a cluster per wave (four waves per block); one split
     // lets each block store its counters.
     const int64_t per_cluster = N / (K > 0 ? K : 1);
