@@ -888,8 +888,9 @@ def run_kernels(args):
     200,000 random rows from HBM, the raw-row gather (impl 2) of the same rows
     from the UNCOMPRESSED table, torch's x[idx] gather, compress_condensed of
     the whole table and the fetch of the same rows from its stream, preprocess
-    and k-means(16) of pinned host data (staged in HBM once), a 256 MiB
-    device copy
+    and k-means(16) of pinned host data (staged in HBM once), k-means(4) and
+    k-means(16) of the whole table and k-means(16) of one 1024-wide and one
+    2752-wide LLM block, a 256 MiB device copy
     (the counter calibration for HBM), the same fetch and raw gather from
     pinned memory, and a sequential read of a 256 MiB pinned table (every row
     in order, impl 2: the calibration for host-read counters)."""
@@ -1006,6 +1007,16 @@ def run_kernels(args):
             if "km" in pm:
                 ok = ok and all(bool(torch.equal(a.cpu(), b.cpu())) for a, b in zip(pm["km"], want_km))
         del x_h
+    for k in (4, 16):
+        # k-means of the whole table (beyond 2^20 rows: the many-block grouping)
+        ev("kmeans_table_k%d" % k, lambda k=k: ibp.preprocess_kmeans(x, k, 0.85, seed=1), x.numel() * 8)
+    if want("kmeans_llm_1024") or want("kmeans_llm_2752"):
+        layer = workloads.llm_weight_blocks(layers=1)[0]
+        for blk in (layer[0], layer[-1]):
+            xb = blk.cuda()
+            ev("kmeans_llm_%d" % xb.shape[1], lambda xb=xb: ibp.preprocess_kmeans(xb, 16, 0.85, seed=1),
+               xb.numel() * 8)
+        del layer, xb
     if not args.no_pcie and want("raw_gather_pinned"):
         x_h = x.cpu().pin_memory()
         ev("raw_gather_pinned", lambda: ibp.decompress_fetch(x_h, mask, bitval, zero, dev, v, idx, None, None, 2,
