@@ -1052,6 +1052,11 @@ int kmeans_typed(const void *data_v, int64_t N, int64_t V, int K, int64_t seed, 
     int32_t *chunk_base = chunked ? chunks + 3 * max_chunks : nullptr;
     int32_t *chunk_count = chunked ? chunk_base + K : nullptr;
     int32_t *nchunks = chunked ? chunk_count + K : nullptr;
+    // Blocks of the many-block histogram and scatter: each adds one global
+    // atomic per cluster it holds, and those on one cluster's counter are
+    // serialised, so two blocks per CU (4096 blocks: 338 us per scatter at
+    // K = 4 on 2.45M rows)
+    const unsigned group_blocks = blocks_for(N, (int64_t)dev.num_cus * 2);
     // chunk sums and majority words in one kernel when every wave's 64
     // counters lie in one cluster
     const bool fused_majority = chunked && (V * B) % kWave == 0;
@@ -1078,12 +1083,12 @@ int kmeans_typed(const void *data_v, int64_t N, int64_t V, int K, int64_t seed, 
                 // more than 2^20 rows: the same grouping and chunk list over
                 // many blocks
                 hipLaunchKernelGGL(km_zero_kernel, dim3(1), dim3(256), 0, s, count, (int64_t)K);
-                hipLaunchKernelGGL(km_hist_kernel, dim3(blocks_for(N)), dim3(256), 0, s, cluster, N, K, count);
+                hipLaunchKernelGGL(km_hist_kernel, dim3(group_blocks), dim3(256), 0, s, cluster, N, K, count);
                 hipLaunchKernelGGL(km_offsets_kernel, dim3(1), dim3(64), 0, s, count, K, offset, fill,
                                    (int)chunk_rows, chunk_base, chunk_count, nchunks);
                 hipLaunchKernelGGL(km_chunks_kernel, dim3(blocks_for(max_chunks, 1 << 20)), dim3(256), 0, s, count,
                                    offset, chunk_base, nchunks, K, (int)chunk_rows, max_chunks, chunks);
-                hipLaunchKernelGGL(km_scatter_kernel, dim3(blocks_for(N)), dim3(256), 0, s, cluster, N, K, offset,
+                hipLaunchKernelGGL(km_scatter_kernel, dim3(group_blocks), dim3(256), 0, s, cluster, N, K, offset,
                                    fill, perm);
             }
             hipLaunchKernelGGL((km_bitcount_chunks_kernel<T>), dim3(tiles, (unsigned)max_chunks), dim3(256), 0, s,
