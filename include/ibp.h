@@ -71,8 +71,9 @@ int ibp_preprocess(int elem_bytes, const void *data, int64_t num_vecs, int64_t v
 /* Replaces preprocess_kmeans (src/preprocess.cu:88-139) -> ibp::preproc_kmeans
  * (include/preproc/ibp_preproc_host.cuh:119-284).  masks/bitvals [K x V]
  * (device), clusters [num_vecs] int32 (device or pinned host).  seed >= 0
- * re-seeds the libc generator with srand(seed) before the random_shuffle
- * initialisation; seed < 0 continues the process-wide rand() state like the
+ * draws the random_shuffle initialisation from the sequence rand() returns
+ * after srand(seed), computed privately (the process-wide generator is not
+ * touched); seed < 0 continues the process-wide rand() state like the
  * reference.  threshold is accepted and ignored, as in the reference.
  * Synchronises the stream before returning. */
 int ibp_preprocess_kmeans(int elem_bytes, const void *data, int64_t num_vecs, int64_t vec_size,
@@ -343,6 +344,10 @@ int ibp_selftest_bitops(int trials, uint64_t seed);
  * release sequences on small arenas, each of which must drain with every
  * request's indices intact.  No GPU work. */
 int ibp_selftest_serve_admission(int trials, uint64_t seed);
+/* Compares the private copy of glibc's rand() sequence that preprocess_kmeans
+ * uses for seed >= 0 with rand() itself for n draws after srand(seed) (this
+ * re-seeds the process-wide generator).  No GPU work. */
+int ibp_selftest_rand(uint32_t seed, int64_t n);
 /* Runs the server's split copy of request indices into pinned memory (the
  * submitting thread and helper threads claiming 64 KiB parts) on host
  * buffers: `trials` copies of random sizes from each of three submitting

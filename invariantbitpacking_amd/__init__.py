@@ -83,8 +83,10 @@ def preprocess_kmeans(dataset, centroids, threshold=None, *, seed=None):
     """masks [K,V], bitvals [K,V] (GPU) and clusters [N] int32 (pinned host).
 
     src/preprocess.cu:88-139.  `threshold` is ignored, as in the reference.
-    `seed` (extension): re-seed libc rand() before the random initialisation;
-    None continues the process-wide rand() state like the reference.
+    `seed` (extension): the random initialisation draws the numbers rand()
+    gives after srand(seed), from a private copy of that sequence (libc's
+    process-wide generator is left untouched); None continues the
+    process-wide rand() state like the reference.
     """
     return _C.preprocess_kmeans(dataset, int(centroids), threshold, seed=seed)
 

@@ -87,6 +87,7 @@ SIGNATURES = {
     "ibp_server_set_budget_override": (_I32, [_P, _SZ]),
     "ibp_server_request_times": (_I32, [_P, _I64, ctypes.POINTER(ctypes.c_double)]),
     "ibp_selftest_bitops": (_I32, [_I32, ctypes.c_uint64]),
+    "ibp_selftest_rand": (_I32, [ctypes.c_uint32, ctypes.c_int64]),
 }
 
 STATUS = {0: "IBP_OK", 1: "IBP_ERR_INVALID", 2: "IBP_ERR_HIP", 3: "IBP_ERR_NOMEM", 4: "IBP_ERR_UNSUPPORTED",

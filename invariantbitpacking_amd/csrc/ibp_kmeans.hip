@@ -891,17 +891,54 @@ float sweep_last_threshold()
     return last;
 }
 
+// The sequence rand() returns after srand(seed) in glibc, without the
+// process-wide generator and its lock: glibc's additive feedback generator
+// (31 words, taps 31 and 3, seeded by the Park-Miller recurrence
+// s[i] = 16807 s[i-1] mod (2^31 - 1), with the first 310 outputs dropped),
+// each output the new word shifted right by one.  ibp_selftest_rand checks it
+// against rand() itself.
+class LibcRandSequence {
+public:
+    explicit LibcRandSequence(unsigned seed)
+    {
+        int32_t s = seed ? (int32_t)seed : 1;
+        ring_[0] = (uint32_t)s;
+        for (int i = 1; i < 31; ++i) {
+            const int64_t hi = s / 127773, lo = s % 127773;
+            int64_t w = 16807 * lo - 2836 * hi; // Schrage's form of 16807 s mod (2^31 - 1)
+            if (w < 0) w += 2147483647;
+            s = (int32_t)w;
+            ring_[i] = (uint32_t)s;
+        }
+        for (int i = 0; i < 310; ++i) (void)next();
+    }
+    int next()
+    {
+        ring_[front_] += ring_[rear_];
+        const uint32_t out = ring_[front_] >> 1;
+        front_ = front_ == 30 ? 0 : front_ + 1;
+        rear_ = rear_ == 30 ? 0 : rear_ + 1;
+        return (int)out;
+    }
+
+private:
+    uint32_t ring_[31];
+    int front_ = 3, rear_ = 0;
+};
+
 // Positions 0..m-1 of libstdc++'s random_shuffle(first, last) over 0..n-1
-// with rand(): every rand() call of the whole shuffle is made, but only the
-// first m positions are kept.  Step i swaps positions i and j <= i, and
-// position i still holds i then (no earlier step reaches past its own
-// index), so for i >= m the step only sets position j to i when j < m.
-std::vector<int32_t> random_shuffle_head(int64_t n, int64_t m)
+// with the generator `rnd` (rand() or its sequence): every draw of the whole
+// shuffle is made, but only the first m positions are kept.  Step i swaps
+// positions i and j <= i, and position i still holds i then (no earlier step
+// reaches past its own index), so for i >= m the step only sets position j
+// to i when j < m.
+template <typename Rnd>
+std::vector<int32_t> random_shuffle_head(int64_t n, int64_t m, Rnd &&rnd)
 {
     std::vector<int32_t> a((size_t)m);
     for (int64_t i = 0; i < m; ++i) a[(size_t)i] = (int32_t)i;
     for (int64_t i = 1; i < n; ++i) {
-        const int64_t j = (int64_t)(rand() % (int)(i + 1));
+        const int64_t j = (int64_t)(rnd() % (int)(i + 1));
         if (i < m) {
             if (i != j) std::swap(a[(size_t)i], a[(size_t)j]);
         } else if (j < m) {
@@ -931,10 +968,18 @@ int kmeans_typed(const void *data_v, int64_t N, int64_t V, int K, int64_t seed, 
     // Host side initialisation (ibp_preproc_host.cuh:150-151,176-188).
     std::vector<int64_t> init((size_t)K, -1);
     if (N > 0) init[0] = 0;
-    if (seed >= 0) srand((unsigned)seed);
     {
+        // seed >= 0: the draws of srand(seed), from a private copy of the
+        // sequence (the process-wide generator is left as it is); otherwise
+        // rand() itself, continuing the process-wide state like the reference
         const int64_t m = K < N ? K : N;
-        const std::vector<int32_t> head = random_shuffle_head(N, m);
+        std::vector<int32_t> head;
+        if (seed >= 0) {
+            LibcRandSequence seq((unsigned)seed);
+            head = random_shuffle_head(N, m, [&]() { return seq.next(); });
+        } else {
+            head = random_shuffle_head(N, m, []() { return rand(); });
+        }
         for (int64_t c = 1; c < m; ++c) init[(size_t)c] = head[(size_t)c];
     }
     T *cent = nullptr, *maj = nullptr, *cent_cols = nullptr;
@@ -1188,4 +1233,17 @@ extern "C" int ibp_preprocess_kmeans(int elem_bytes, const void *data, int64_t n
         dprintf("Finished clustering\n");
     }
     return st;
+}
+
+extern "C" int ibp_selftest_rand(uint32_t seed, int64_t n)
+{
+    IBP_CHECK(n >= 0, "n must be >= 0");
+    LibcRandSequence seq(seed);
+    srand(seed);
+    for (int64_t i = 0; i < n; ++i) {
+        const int want = rand(), got = seq.next();
+        IBP_CHECK(want == got, "draw %lld after srand(%u): rand() gave %d, the private sequence %d", (long long)i,
+                  seed, want, got);
+    }
+    return IBP_OK;
 }

@@ -301,8 +301,9 @@ std::tuple<Tensor, Tensor> preprocess(const Tensor &dataset, std::optional<doubl
 
 // preprocess_kmeans (src/preprocess.cu:88-139): masks, bitvals [K x V] on the
 // GPU and clusters [N] int32 in pinned host memory.  threshold is ignored, as
-// in the reference.  seed (extension) re-seeds libc rand() first; None keeps
-// the process-wide state.  Synchronises.
+// in the reference.  seed (extension) draws the initialisation from the
+// sequence of srand(seed) without touching libc's generator; None continues
+// the process-wide rand() state.  Synchronises.
 std::tuple<Tensor, Tensor, Tensor> preprocess_kmeans(const Tensor &dataset, int64_t num_clusters,
                                                      std::optional<double> threshold, std::optional<int64_t> seed)
 {

@@ -42,6 +42,14 @@ def test_selftest_bitops_host(capi):
     assert capi.lib().ibp_selftest_bitops(50000, 12345) == 0, capi.last_error()
 
 
+def test_selftest_rand_sequence_host(capi):
+    """The private copy of glibc's rand() sequence (preprocess_kmeans with a
+    seed) against rand() itself, seeds 0 (which glibc treats as 1), 1, a
+    large one and 2^32 - 1."""
+    for seed in (0, 1, 3, 12345, 2**32 - 1):
+        assert capi.lib().ibp_selftest_rand(seed, 200000) == 0, capi.last_error()
+
+
 def test_selftest_serve_admission_host(capi):
     """The server's admission bookkeeping with a mock launcher: the queue head
     is admitted by taking back later requests' pinned index space, and random
