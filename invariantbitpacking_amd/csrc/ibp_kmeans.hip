@@ -553,7 +553,7 @@ __global__ void km_offsets_kernel(const int32_t *count, int K, int32_t *offset, 
         fill[c] = 0;
         acc += count[c];
         if (chunk_base) {
-            const int32_t nc = (count[c] + chunk_rows - 1) / chunk_rows;
+            const int32_t nc = (int32_t)(((int64_t)count[c] + chunk_rows - 1) / chunk_rows);
             chunk_base[c] = e;
             chunk_count[c] = nc;
             e += nc;
@@ -579,11 +579,11 @@ __global__ void km_chunks_kernel(const int32_t *count, const int32_t *offset, co
             hi = mid - 1;
     }
     const int c = lo;
-    const int32_t b = (int32_t)(e - chunk_base[c]) * chunk_rows;
-    const int32_t n = count[c];
+    const int64_t b = (e - chunk_base[c]) * (int64_t)chunk_rows;
+    const int64_t n = count[c];
     chunks[3 * e] = c;
-    chunks[3 * e + 1] = offset[c] + b;
-    chunks[3 * e + 2] = offset[c] + (b + chunk_rows < n ? b + chunk_rows : n);
+    chunks[3 * e + 1] = (int32_t)(offset[c] + b);
+    chunks[3 * e + 2] = (int32_t)(offset[c] + (b + chunk_rows < n ? b + chunk_rows : n));
 }
 
 // perm[offset[c] ..) = the rows of cluster c.  A block counts its rows per
