@@ -32,7 +32,9 @@ using WideT = typename std::conditional<sizeof(T) == 8, uint64_t, uint32_t>::typ
 // Column-bit counts of the row block blockIdx.x and the 64-column tile
 // blockIdx.y: part[blockIdx.x][k*V + j] = rows of the block whose column j has
 // bit k set.  Lane <-> column; the block's waves split the rows; every thread
-// keeps its B counters in registers.  Rows are taken 16 at a time (16 loads in
+// keeps its B counters in registers.  Rows of fewer than 32 elements take
+// G = 64 / V rows per load (lane l: column l % V of row l / V, so a load reads
+// G * V consecutive elements), and the G lanes of a column meet in the LDS sum.  Rows are taken 16 at a time (16 loads in
 // flight) and added into bit-sliced counters (VCounter), folded into the B
 // counters every 240 rows.  The waves' counters are summed in LDS and stored
 // once per block (no global atomics: every block writes its own partial
@@ -48,8 +50,11 @@ __global__ __launch_bounds__(256) void count_bits_kernel(const T *data, int64_t 
     const int lane = lane_id();
     const int wave_in_block = wave_in_block_id();
     const int waves = blockDim.x / kWave;
-    const int64_t j = (int64_t)blockIdx.y * kWave + lane;
-    const bool col = j < V;
+    const int G = V < kWave / 2 ? (int)(kWave / V) : 1; // rows per load
+    const int lcol = G > 1 ? lane % (int)V : lane;
+    const int lrow = G > 1 ? lane / (int)V : 0;
+    const int64_t j = (int64_t)blockIdx.y * kWave + lcol;
+    const bool col = j < V && lrow < G;
     const int64_t r0 = (int64_t)blockIdx.x * rows_per_block;
     int64_t r1 = r0 + rows_per_block;
     if (r1 > num_vecs) r1 = num_vecs;
@@ -58,11 +63,12 @@ __global__ __launch_bounds__(256) void count_bits_kernel(const T *data, int64_t 
 #pragma unroll
     for (int k = 0; k < B; ++k) c[k] = 0;
     VCounter<W, B> vc;
-    for (int64_t r = r0 + wave_in_block; r < r1; r += (int64_t)CHUNK * waves) {
+    const int64_t step = (int64_t)waves * G; // rows between a wave's loads
+    for (int64_t r = r0 + (int64_t)wave_in_block * G; r < r1; r += CHUNK * step) {
         W x[CHUNK];
 #pragma unroll
         for (int q = 0; q < CHUNK; ++q) {
-            const int64_t rr = r + (int64_t)q * waves;
+            const int64_t rr = r + (int64_t)q * step + lrow;
             x[q] = (col && rr < r1) ? (W)data[rr * V + j] : W(0);
         }
         vc.add16(x, c);
@@ -71,7 +77,7 @@ __global__ __launch_bounds__(256) void count_bits_kernel(const T *data, int64_t 
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < B; ++k)
-        if (c[k]) atomicAdd(&sum[k * kWave + lane], c[k]);
+        if (c[k]) atomicAdd(&sum[k * kWave + lcol], c[k]);
     __syncthreads();
     int32_t *out = part + (int64_t)blockIdx.x * V * B;
     for (int i = threadIdx.x; i < B * kWave; i += blockDim.x) {
