@@ -64,6 +64,50 @@ __global__ __launch_bounds__(256) void size_kernel(const T *data, int64_t num_ve
     const int64_t wave = global_wave_id();
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
     unsigned long long acc = 0, bad = 0;
+    if (V < kWave / 2 && clusters == nullptr) {
+        // Rows of fewer than 32 elements with one mask: G = 64 / V rows per
+        // step, lane l on column l % V of row l / V (consecutive rows: one
+        // load reads G * V consecutive elements).  A row's saved bits are the
+        // wave's inclusive scan at its last lane minus the scan before its
+        // first; the row's first lane stores its size.
+        using W = typename std::conditional<sizeof(T) == 8, uint64_t, uint32_t>::type;
+        const int Vi = (int)V;
+        const int G = kWave / Vi;
+        const int lrow = lane / Vi, lcol = lane % Vi;
+        const bool on = lrow < G;
+        const T m = on ? mask[lcol] : T(0), b = on ? bitval[lcol] : T(1);
+        const int pc = on ? popc((W)m) : 0; // lanes past the rows never match (b has a bit outside m)
+        const int first = lrow * Vi, last = first + Vi - 1;
+        for (int64_t i0 = wave * G; i0 < n; i0 += nwaves * G) {
+            const int64_t i = i0 + lrow;
+            const bool live = on && i < n;
+            const int64_t r = live ? row_of<IdxT>(index, i) : -1;
+            const bool ok = r >= 0 && r < num_vecs;
+            const T x = ok ? data[r * V + lcol] : T(0);
+            const int incl = wave_incl_scan(ok && (T)(x & m) == b ? pc : 0);
+            // both exchanges run in every lane: ds_bpermute returns 0 from a
+            // lane that is not active
+            const int hi = (int)shfl((uint32_t)incl, last & (kWave - 1));
+            const int before = (int)shfl((uint32_t)incl, (first + kWave - 1) & (kWave - 1));
+            const int lo = first > 0 ? before : 0;
+            if (live && lcol == 0) {
+                const int64_t size = ok ? slot_size(V, (int)sizeof(T), hi - lo) : 0;
+                sizes[i] = size;
+                acc += (unsigned long long)size;
+                bad += ok ? 0 : 1;
+            }
+        }
+        // the wave's totals: 24-bit low parts and the rest, each summed in 32 bits
+        const unsigned long long a =
+            (unsigned long long)(uint32_t)wave_sum((int)(uint32_t)(acc & 0xffffffull)) +
+            ((unsigned long long)(uint32_t)wave_sum((int)(uint32_t)(acc >> 24)) << 24);
+        const unsigned long long e = (unsigned long long)(uint32_t)wave_sum((int)bad);
+        if (lane == 0) {
+            if (total && a) atomicAdd(total, a);
+            if (err && e) atomicAdd(err, e);
+        }
+        return;
+    }
     if (V <= kWave && clusters == nullptr) {
         // Rows of <= 64 elements with one mask: the lane's column constants
         // stay in registers and R rows are in flight per wave.

@@ -278,6 +278,34 @@ def test_golden_slots_on_gpu(ibp):
         assert np.array_equal(_np(out), row), c["name"]
 
 
+@pytest.mark.parametrize("w,v", [(8, 8), (8, 3), (4, 5), (2, 17), (1, 31)])
+def test_size_narrow_rows_with_index_and_counter(ibp, oracle, w, v):
+    """Rows of fewer than 32 elements (several rows per wave step), whole
+    tables and indices with repeats and out-of-range entries, against the
+    oracle; the counter adds the sizes of the valid rows."""
+    data = float_rows(1000, v, w, seed=v + w)
+    om, ob, _, _ = oracle.preprocess(data, 0.9)
+    dt = np.dtype("i%d" % w)
+    mask, bitval = (torch.from_numpy(a.view(dt)).cuda() for a in (om, ob))
+    full, _ = oracle.get_compress_size(data, om, ob)
+    idx = np.array([5, 0, 999, 5, 12, 77, 3, 998, 500], dtype=np.int64)
+    osz, _ = oracle.get_compress_size(data, om, ob, idx)
+    for where in ("device", "pinned"):
+        t = place(data, where)
+        whole = ibp.get_compress_size(t, mask, bitval)
+        torch.cuda.synchronize()  # asynchronous, as in the reference (pinned output for a pinned table)
+        assert np.array_equal(_np(whole), full)
+        ctr = torch.tensor([100], dtype=torch.int64, device="cuda")
+        sz = ibp.get_compress_size(t, mask, bitval, torch.from_numpy(idx).cuda(), ctr)
+        torch.cuda.synchronize()
+        assert np.array_equal(_np(sz), osz)
+        assert int(ctr[0]) == 100 + int(osz.sum())
+    bad = torch.tensor([1, -1, 1000, 2], dtype=torch.int64, device="cuda")
+    sz = ibp.get_compress_size(place(data, "device"), mask, bitval, bad)
+    torch.cuda.synchronize()
+    assert _np(sz)[1] == 0 and _np(sz)[2] == 0 and _np(sz)[0] == full[1] and _np(sz)[3] == full[2]
+
+
 def test_size_with_index_and_counter(ibp, oracle):
     data = float_rows(400, 64, 8, seed=3)
     om, ob, _, _ = oracle.preprocess(data, 0.9)
